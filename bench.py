@@ -1,0 +1,197 @@
+"""Batch-OMP signals/s and K-SVD iterations/s on one MI355X.
+
+Workload (fits one GPU): the precision-form configuration, n=256 signal
+length, K=1024 atoms, N=1,000,000 signals, T=32 atoms per signal, Gaussian
+unit-norm atoms with 25 dB SNR, synthetic and seeded, resident in HBM before
+timing starts.  A step is one Batch-OMP pass over all N signals:
+G = D^T D, DtX = D^T X (MFMA, --form), then the Batch-OMP kernel.
+
+    python bench.py [--steps 10] [--warmup 2] [--form bf16x3] [--gpus 1]
+
+Prints ONE JSON line with the metric, roofline of the dominant kernel (the
+Batch-OMP kernel) and the CPU baseline (the FP32 port of the reference
+algorithm in oracle/, timed on a bounded sample of the same signals).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Batch-OMP signals/sec + K-SVD iters/sec at n=256,K=1024,T=32; % roofline"
+FP32_PEAK_TF = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def omp_flops_per_signal(K, T):
+    # alpha update sum_{i<=T} 2 K i + ~3 i^2 per step for the Cholesky row and two solves
+    return K * T * (T + 1) + T * (T + 1) * (2 * T + 1) / 2
+
+
+def cpu_baseline(X_sample, D, T, threads):
+    """FP32 port (oracle/liboracle_f32.so) of the reference Batch-OMP, OpenMP
+    over signals: G, DtX and Batch-OMP on the sample."""
+    import numpy as np
+
+    import oracle
+    o = oracle.load("f32")
+    used = o.set_threads(threads)
+    Xs = np.ascontiguousarray(X_sample, dtype=np.float32)
+    Dn = np.ascontiguousarray(D, dtype=np.float32)
+    t0 = time.perf_counter()
+    G = o.gram(Dn)
+    A0 = o.correlate(Dn, Xs)
+    o.batch_omp(A0, G, T)
+    dt = time.perf_counter() - t0
+    return Xs.shape[0] / dt, used, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form", default="bf16x3", choices=["fp32", "bf16", "bf16x3"])
+    ap.add_argument("--N", type=int, default=1_000_000)
+    ap.add_argument("--ksvd-iters", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the engine has no multi-GPU path (see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from ksvd_amd import Engine, synth
+
+    n, K, T = 256, 1024, 32
+    N = args.N
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log("generating workload N =", N)
+    wl = synth.make("C2", device=dev, N=N)
+    X, D0 = wl.X.contiguous(), wl.D_init.contiguous()
+    del wl
+    torch.cuda.synchronize()
+
+    eng = Engine(n, K, N, T, gemm_form=args.form)
+    D = eng.normcols(D0)
+    G = torch.empty((K, K), dtype=torch.float32, device=dev)
+    A0 = torch.empty((N, K), dtype=torch.float32, device=dev)
+    idx = torch.empty((N, T), dtype=torch.int32, device=dev)
+    val = torch.empty((N, T), dtype=torch.float32, device=dev)
+    cnt = torch.empty((N,), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    omp_ms, gemm_ms = [], []
+
+    def step(record):
+        if record:
+            ev[0].record(stream)
+        eng.gram(D, G)
+        eng.correlate(D, X, A0)
+        if record:
+            ev[1].record(stream)
+        eng.batch_omp_compact(A0, G, (idx, val, cnt))
+        if record:
+            ev[2].record(stream)
+
+    log("warmup", args.warmup)
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    log("timing", args.steps, "steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+        torch.cuda.synchronize()  # per-step events are read below; sync keeps them valid
+        gemm_ms.append(ev[0].elapsed_time(ev[1]))
+        omp_ms.append(ev[1].elapsed_time(ev[2]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    value = N / (ms_per_step / 1000.0)
+    omp_avg = float(np.mean(omp_ms))
+    c = cnt.cpu().numpy()
+    log(f"step {ms_per_step:.2f} ms (gemms {np.mean(gemm_ms):.2f} ms, omp {omp_avg:.2f} ms), "
+        f"{value / 1e6:.2f} M signals/s, cnt mean {c.mean():.2f}")
+
+    # K-SVD iterations (G, Batch-OMP, dictionary update, error, clearDict)
+    ksvd_ms = None
+    if args.ksvd_iters > 0:
+        Dout = torch.empty_like(D0)
+        eng.ExecuteIterations(X, D0, 1, D_out=Dout, out=(idx, val, cnt))  # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, _, _, _, rmse = eng.ExecuteIterations(X, D0, args.ksvd_iters, D_out=Dout, out=(idx, val, cnt))
+        torch.cuda.synchronize()
+        ksvd_ms = (time.perf_counter() - t1) * 1000.0 / args.ksvd_iters
+        log(f"K-SVD iteration {ksvd_ms:.1f} ms, rmse {rmse}")
+
+    flops = omp_flops_per_signal(K, T) * N
+    achieved_tf = flops / (omp_avg / 1000.0) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved_tf, 3),
+        "peak": FP32_PEAK_TF,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tf / FP32_PEAK_TF, 4),
+        "traffic": None,
+        "kernel": "omp_wave_kernel<16,false>",
+        "kernel_ms": round(omp_avg, 3),
+        "work_per_launch": f"{omp_flops_per_signal(K, T):.0f} FP32 FLOP/signal x {N} signals",
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        S = min(args.cpu_sample, N)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        Xs = X[:S].cpu().numpy()
+        Dh = D.cpu().numpy()
+        log(f"cpu baseline on {S} signals, {threads} threads")
+        sps, used, dt = cpu_baseline(Xs, Dh, T, threads)
+        cpu = {"value": round(sps, 1), "unit": "signals/s", "cores": used, "kind": "port",
+               "sample": f"first {S} of the {N} signals: G, DtX and Batch-OMP in FP32 "
+                         f"(oracle/liboracle_f32.so, OpenMP), {dt:.1f} s"}
+        log(f"cpu {sps:.0f} signals/s ({dt:.1f} s)")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "signals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "C2 precision form: n=256 K=1024 N=1000000 T=32, Gaussian unit atoms, 25 dB SNR",
+                   "n": n, "K": K, "N": N, "T": T, "gemm_form": args.form, "parallelism": "single GPU"},
+        "ksvd_iters_per_s": round(1000.0 / ksvd_ms, 3) if ksvd_ms else None,
+        "ksvd_ms_per_iter": round(ksvd_ms, 2) if ksvd_ms else None,
+        "phase_ms": {"gemms": round(float(np.mean(gemm_ms)), 3), "batch_omp": round(omp_avg, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * ksvd_amd.h -- C-ABI of the MI355X K-SVD engine (libksvd_amd.so).
+ *
+ * Drop-in boundary for the reference's Batch-OMP / K-SVD path.  Each entry
+ * point names the reference interface it replaces (S/ = K-SVD/K-SVD/ in the
+ * reference tree).  The reference exposes C++ classes over a global
+ * DeviceMemory arena; here every buffer is caller-owned device memory passed
+ * as a plain pointer, with the reference's layouts:
+ *
+ *   X  n x N  column-major float   (signal s = X + s*n)        X_ARRAY
+ *   D  n x K  column-major float   (atom k   = D + k*n)        TEMP_ROWSD_BY_COLSD
+ *   G  K x K  column-major float                               G
+ *   DtX K x N column-major float   (alpha0 of signal s)        DtX
+ *   SELECTED_ATOMS K x N float: slot+1 (1..T) or 0; after the
+ *     dictionary update, the dense Gamma values                SELECTED_ATOMS
+ *   c  T x N float: coefficients in selection order             c
+ *   (S/DeviceMemory.cuh:36-64, S/DeviceMemory.cu:55-137)
+ *
+ * The engine's own fast layout is the compact Gamma: idx (T x N, int32, -1
+ * padded), val (T x N, float, 0 padded), cnt (N, int32).
+ *
+ * Conventions: every call returns KSVD_OK (0) or a negative status;
+ * ksvd_last_error() gives the text.  Calls are asynchronous on the handle's
+ * stream unless stated; the handle owns its scratch memory.  No globals:
+ * one handle per host thread.
+ */
+#ifndef KSVD_AMD_H
+#define KSVD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ksvd_handle ksvd_handle;
+
+enum {
+    KSVD_OK = 0,
+    KSVD_ERR_INVALID = -1, /* bad argument or shape */
+    KSVD_ERR_LIMIT = -2,   /* size beyond what the kernels support */
+    KSVD_ERR_HIP = -3,     /* HIP runtime error */
+    KSVD_ERR_ALLOC = -4,   /* device allocation failed */
+    KSVD_ERR_NULL = -5     /* required pointer is NULL */
+};
+
+/* Precision form of the two GEMMs (G = D^T D, DtX = D^T X). */
+enum {
+    KSVD_FORM_FP32 = 0,   /* FP32 operands, FP32 MFMA, FP32 accumulate      */
+    KSVD_FORM_BF16 = 1,   /* BF16 operands on MFMA, FP32 accumulate         */
+    KSVD_FORM_BF16X3 = 2  /* split-BF16 x3: hi*hi + hi*lo + lo*hi, FP32 acc */
+};
+
+/* Batch-OMP kernel variant. */
+enum {
+    KSVD_OMP_AUTO = 0,  /* choose from the autotuner cache / heuristics */
+    KSVD_OMP_WAVE = 1   /* one wave64 per signal, alpha in VGPRs, L in LDS */
+};
+
+/*
+ * Runtime configuration.  Replaces the reference's compile-time parameters
+ * (S/ProgramParameters.cuh:17-23, S/GlobalDeclarations.cuh:27-63) and the
+ * dimensions read from MAT files (S/Globals.cpp:10-37).
+ */
+typedef struct ksvd_config {
+    int32_t n;            /* signal length  = rowsX = rowsD                     */
+    int32_t K;            /* atoms          = colsD                             */
+    int32_t N;            /* signals        = colsX                             */
+    int32_t T;            /* sparsity Tdata (T_max in error mode), <= 64        */
+    int32_t iterations;   /* NUMBERofITERATIONS                                 */
+    int32_t use_thresh;   /* USE_THRESH (4)                                     */
+    int32_t max_signals;  /* MAX_SIGNALS (5000)                                 */
+    int32_t gemm_form;    /* KSVD_FORM_*                                        */
+    int32_t omp_variant;  /* KSVD_OMP_*                                         */
+    int32_t reserved0;
+    double mu_thresh;     /* muTHRESH (0.8)                                     */
+    double pivot_fp64;    /* Cholesky pivot 1-||w||^2 below this is recomputed  */
+                          /* in FP64 (0 disables the fallback)                  */
+    uint64_t perm_seed;   /* seed of the atom-order permutation stream          */
+} ksvd_config;
+
+/* Fills the reference defaults (T=6, 30 iterations, mu=0.8, use_thresh=4,
+ * max_signals=5000), FP32 form, pivot_fp64 = 1e-3, perm_seed = 0. */
+void ksvd_config_default(ksvd_config* cfg);
+
+/* Handle lifetime.  Replaces the KSVD/CudaAlgorithm constructors
+ * (S/KSVD.cu:42-44, S/CudaAlgorithm.cu:23-33) and DeviceMemory set-up. */
+int ksvd_create(ksvd_handle** h, const ksvd_config* cfg);
+int ksvd_destroy(ksvd_handle* h);
+/* stream is a hipStream_t (NULL = default stream). */
+int ksvd_set_stream(ksvd_handle* h, void* stream);
+const char* ksvd_last_error(const ksvd_handle* h);
+const char* ksvd_version(void);
+/* Blocks until the handle's stream is idle. */
+int ksvd_sync(ksvd_handle* h);
+
+/* ---- primitives ---------------------------------------------------------- */
+
+/* D_out = D_in * diag(1/||d_k||).  KSVD::normcols  S/KSVD.cu:267-286. */
+int ksvd_normcols(ksvd_handle* h, const float* D_in, float* D_out);
+
+/* C (colsA x colsB) = A^T B, column-major, lda = ldb = rows, in the handle's
+ * GEMM form.  Algorithms::Multiplication::AT_times_B  S/Multiplication.cu:40-63. */
+int ksvd_at_times_b(ksvd_handle* h, const float* A, int rows, int colsA, const float* B, int colsB,
+                    float* C);
+
+/* G = D^T D (K x K).  AT_times_B call at S/KSVD.cu:64-73. */
+int ksvd_gram(ksvd_handle* h, const float* D, float* G);
+
+/* DtX = D^T X (K x N).  AT_times_B call at S/OMP.cu:44-53. */
+int ksvd_correlate(ksvd_handle* h, const float* D, const float* X, float* DtX);
+
+/* ---- Batch-OMP ----------------------------------------------------------- */
+
+/* Reference-layout Batch-OMP: reads DtX and G, writes SELECTED_ATOMS (slot map)
+ * and c.  Slots of c beyond a signal's accepted count are left untouched, as in
+ * the reference.  parallelOMP / parOMP  S/OMP.cu:100-113, S/OMP.cu:254-405. */
+int ksvd_batch_omp(ksvd_handle* h, const float* DtX, const float* G, float* selected_atoms,
+                   float* c);
+
+/* Same algorithm, compact output. */
+int ksvd_batch_omp_compact(ksvd_handle* h, const float* DtX, const float* G, int32_t* idx,
+                           float* val, int32_t* cnt);
+
+/* OMP::BatchOMP  S/OMP.cu:42-59: DtX = D^T X in the handle's GEMM form (into
+ * handle scratch) followed by Batch-OMP; compact output. */
+int ksvd_batch_omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G,
+                                int32_t* idx, float* val, int32_t* cnt);
+
+/* Error-constrained Batch-OMP (no reference counterpart beyond the comment at
+ * S/KSVD.cu:79): per signal, stop when ||r||^2 = ||x||^2 - alpha0_I^T gamma_I
+ * (evaluated in FP64) <= eps2[s], or at T atoms (T_max).  xnorm2[s] = ||x_s||^2.
+ * Runs as one persistent launch with work stealing over signals. */
+int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const float* xnorm2,
+                       const float* eps2, int32_t* idx, float* val, int32_t* cnt);
+
+/* ---- dictionary update / error / clear ------------------------------------ */
+
+/* Resets the unused-signal list, bitmap, replaced flags and counter.
+ * initialize()/init  S/KSVD.cu:368-375, S/KSVD.cu:442-457. */
+int ksvd_du_init(ksvd_handle* h);
+
+/* One approximate K-SVD sweep over atoms in the order perm[0..K-1] (a HOST
+ * pointer, as in the reference).  D and val are updated in place; idx/cnt are
+ * read only.  DictionaryUpdate::updateDictionary  S/DictionaryUpdate.cu:45-203. */
+int ksvd_dictionary_update(ksvd_handle* h, const float* X, float* D, const int32_t* idx,
+                           float* val, const int32_t* cnt, const uint32_t* perm_host);
+
+/* Reference-layout variant: selected_atoms holds the slot map on entry and the
+ * dense Gamma values on exit (sprow_kernel S/DictionaryUpdate.cu:378-395); c is
+ * read.  The pivot-break slot (marked but never written by parOMP) reads the
+ * stale c entry, as the reference does. */
+int ksvd_dictionary_update_dense(ksvd_handle* h, const float* X, float* D, float* selected_atoms,
+                                 const float* c, const uint32_t* perm_host);
+
+/* err[s] = ||x_s - D gamma_s||^2 masked by the unused-signal bitmap; *rmse_host
+ * (if not NULL) receives sqrt(sum of unmasked err / (n N)) -- this call then
+ * synchronises.  KSVD::errorComputation  S/KSVD.cu:235-262. */
+int ksvd_error(ksvd_handle* h, const float* X, const float* D, const int32_t* idx,
+               const float* val, const int32_t* cnt, float* err, double* rmse_host);
+
+/* Replaces atoms that are too coherent with another atom (|d_i^T d_j| > mu) or
+ * used by fewer than use_thresh signals, with the worst-represented signal.
+ * KSVD::clearDict  S/KSVD.cu:138-226. */
+int ksvd_clear_dict(ksvd_handle* h, const float* X, float* D, const int32_t* idx,
+                    const float* val, const int32_t* cnt, float* err);
+
+/* Full loop: normcols(D_in) -> D_out, then per iteration G, Batch-OMP, init,
+ * permutation, dictionary update, error, clearDict.  rmse_host[it] receives
+ * the per-iteration RMSE (host array of `iterations` doubles, may be NULL).
+ * KSVD::ExecuteIterations  S/KSVD.cu:55-131. */
+int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, float* D_out,
+                            int32_t* idx, float* val, int32_t* cnt, int iterations,
+                            double* rmse_host);
+
+/* Host-only (no device work): the atom-order permutations ksvd_execute_iterations
+ * uses -- iterations x K values, row it = the running buffer after it+1
+ * shuffles.  Replaces Algorithms::RandomPermutation::generateRandPerm
+ * (S/RandPermutation.cu:35-42: std::random_shuffle of a running buffer) with a
+ * seeded SplitMix64 Fisher-Yates shuffle of the same running buffer. */
+int ksvd_perm_sequence(uint64_t seed, int K, int iterations, uint32_t* out);
+
+/* ---- differentiation ----------------------------------------------------- */
+
+/* Adjoint of the fixed-support least squares gamma_I = (D_I^T D_I)^{-1} D_I^T x
+ * (no reference counterpart).  gval (T x N) = dL/dgamma in slot order.
+ * Writes gX (n x N) = D_I z and gD (n x K) = sum_s (r z^T - D_I z gamma^T)
+ * scattered to the support columns, z = (D_I^T D_I)^{-1} g, r = x - D_I gamma. */
+int ksvd_omp_adjoint(ksvd_handle* h, const float* D, const float* X, const int32_t* idx,
+                     const float* val, const int32_t* cnt, const float* gval, float* gD,
+                     float* gX);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KSVD_AMD_H */

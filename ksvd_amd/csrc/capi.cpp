@@ -1,0 +1,434 @@
+// capi.cpp -- the C-ABI of libksvd_amd.so (declared in include/ksvd_amd.h):
+// handle, configuration checks, scratch ownership and the host-side driver
+// loop.  Replaces the reference's KSVD / OMP / DictionaryUpdate classes over
+// the DeviceMemory arena (S/KSVD.cuh, S/OMP.cuh, S/DictionaryUpdate.cuh,
+// S/DeviceMemory.cu) with caller-owned buffers and a per-handle stream.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ksvd_amd.h"
+#include "common.hpp"
+
+namespace ksvd {
+hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t st);
+}
+
+struct ksvd_handle {
+    ksvd_config cfg;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<void*> allocs;
+    ksvd::DuWork du{};
+    bool du_ready = false;
+    float* dtx = nullptr;      // K x N correlation scratch
+    float* gram = nullptr;     // K x K
+    float* errv = nullptr;     // N
+    int32_t* counter = nullptr;
+    int32_t* cidx = nullptr;   // compact scratch (dense entry points)
+    float* cval = nullptr;
+    int32_t* ccnt = nullptr;
+    double* rmse_all = nullptr;
+    int rmse_cap = 0;
+    uint32_t* perm_all = nullptr;
+    int perm_cap = 0;
+};
+
+namespace {
+
+int fail(ksvd_handle* h, int code, const char* fmt, ...) {
+    if (h) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        h->err = buf;
+    }
+    return code;
+}
+
+int hip_check(ksvd_handle* h, hipError_t e, const char* what) {
+    if (e == hipSuccess) return KSVD_OK;
+    return fail(h, KSVD_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+template <typename T>
+int alloc(ksvd_handle* h, T** p, size_t count, const char* what) {
+    if (*p) return KSVD_OK;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T) + 256);
+    if (e != hipSuccess) return fail(h, KSVD_ERR_ALLOC, "hipMalloc(%s, %zu bytes): %s", what, count * sizeof(T),
+                                     hipGetErrorString(e));
+    h->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return KSVD_OK;
+}
+
+#define TRY(x)                        \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != KSVD_OK) return rc_; \
+    } while (0)
+#define HIPTRY(h, x, what) TRY(hip_check((h), (x), (what)))
+
+int check_cfg(ksvd_handle* h, const ksvd_config* c) {
+    if (c->n < 1 || c->K < 1 || c->N < 1 || c->T < 1)
+        return fail(h, KSVD_ERR_INVALID, "n, K, N, T must be positive (n=%d K=%d N=%d T=%d)", c->n, c->K, c->N, c->T);
+    if (c->T > 64) return fail(h, KSVD_ERR_LIMIT, "T=%d exceeds 64 (one support slot per lane)", c->T);
+    if (c->T > c->K) return fail(h, KSVD_ERR_INVALID, "T=%d exceeds K=%d", c->T, c->K);
+    if (c->K > 2048) return fail(h, KSVD_ERR_LIMIT, "K=%d exceeds 2048 (alpha held in VGPRs)", c->K);
+    if (c->n > 4096) return fail(h, KSVD_ERR_LIMIT, "n=%d exceeds 4096", c->n);
+    if ((long)c->N * c->T > 0x7fffffffL) return fail(h, KSVD_ERR_LIMIT, "N*T exceeds 2^31");
+    if (c->gemm_form < 0 || c->gemm_form > 2) return fail(h, KSVD_ERR_INVALID, "gemm_form=%d", c->gemm_form);
+    if (c->mu_thresh < 0) return fail(h, KSVD_ERR_INVALID, "mu_thresh must be >= 0");
+    if (c->max_signals < 1) return fail(h, KSVD_ERR_INVALID, "max_signals must be >= 1");
+    return KSVD_OK;
+}
+
+int ensure_du(ksvd_handle* h) {
+    if (h->du_ready) return KSVD_OK;
+    ksvd::DuWork& w = h->du;
+    const ksvd_config& c = h->cfg;
+    w.n = c.n; w.K = c.K; w.N = c.N; w.T = c.T;
+    w.max_signals = c.max_signals;
+    w.use_thresh = c.use_thresh;
+    w.mu_thresh = c.mu_thresh;
+    w.chunk = 256;
+    w.nchunks = (c.N + w.chunk - 1) / w.chunk;
+    w.vparts = 64;
+    w.rmse_parts = 2048;
+    const int M = c.N < c.max_signals ? c.N : c.max_signals;
+    TRY(alloc(h, &w.count, c.K, "count"));
+    TRY(alloc(h, &w.ptr, c.K + 1, "ptr"));
+    TRY(alloc(h, &w.chunk_hist, (size_t)c.K * w.nchunks, "chunk_hist"));
+    TRY(alloc(h, &w.list_pos, (size_t)c.N * c.T, "list_pos"));
+    TRY(alloc(h, &w.gn2, c.K, "gn2"));
+    TRY(alloc(h, &w.Y, (size_t)c.n * c.K, "Y"));
+    TRY(alloc(h, &w.vpart, (size_t)w.vparts * c.K, "vpart"));
+    TRY(alloc(h, &w.avec, c.n, "avec"));
+    TRY(alloc(h, &w.uvec, c.K, "uvec"));
+    TRY(alloc(h, &w.scal, 8, "scal"));
+    TRY(alloc(h, &w.err_cand, M, "err_cand"));
+    TRY(alloc(h, &w.unused, c.N, "unused"));
+    TRY(alloc(h, &w.us_counter, 1, "us_counter"));
+    TRY(alloc(h, &w.bitmap, c.N, "bitmap"));
+    TRY(alloc(h, &w.replaced, c.K, "replaced"));
+    TRY(alloc(h, &w.perm_dev, c.K, "perm"));
+    TRY(alloc(h, &w.usecount, c.K, "usecount"));
+    TRY(alloc(h, &w.Gtmp, (size_t)c.K * c.K, "Gtmp"));
+    TRY(alloc(h, &w.rmse_part, w.rmse_parts, "rmse_part"));
+    TRY(alloc(h, &w.rmse_dev, 1, "rmse"));
+    TRY(alloc(h, &w.repl_list, c.K + 1, "repl_list"));
+    h->du_ready = true;
+    HIPTRY(h, ksvd::launch_du_init(w, h->stream), "du_init");
+    return KSVD_OK;
+}
+
+// SplitMix64 (Steele, Lea & Flood 2014) driving a Fisher-Yates shuffle
+// (Knuth TAOCP 3.4.2 Algorithm P) of a running buffer, one shuffle per
+// iteration -- the running-buffer semantics of
+// Algorithms::RandomPermutation::generateRandPerm (S/RandPermutation.cu:35-42)
+// with a seeded, documented generator instead of std::random_shuffle.
+uint64_t splitmix64(uint64_t* st) {
+    uint64_t z = (*st += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+void shuffle(uint32_t* buf, int K, uint64_t* st) {
+    for (int i = K - 1; i >= 1; --i) {
+        const int j = (int)(splitmix64(st) % (uint64_t)(i + 1));
+        const uint32_t t = buf[i];
+        buf[i] = buf[j];
+        buf[j] = t;
+    }
+}
+
+ksvd::OmpArgs omp_args(ksvd_handle* h, const float* DtX, const float* G) {
+    ksvd::OmpArgs a{};
+    a.DtX = DtX;
+    a.G = G;
+    a.K = h->cfg.K;
+    a.N = h->cfg.N;
+    a.T = h->cfg.T;
+    a.pivot_fp64 = (float)h->cfg.pivot_fp64;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ksvd_config_default(ksvd_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->T = 6;
+    c->iterations = 30;
+    c->use_thresh = 4;
+    c->max_signals = 5000;
+    c->gemm_form = KSVD_FORM_FP32;
+    c->omp_variant = KSVD_OMP_AUTO;
+    c->mu_thresh = 0.8;
+    c->pivot_fp64 = 1e-3;
+    c->perm_seed = 0;
+}
+
+const char* ksvd_version(void) { return "ksvd_amd 0.1 (gfx950)"; }
+
+int ksvd_create(ksvd_handle** out, const ksvd_config* cfg) {
+    if (!out || !cfg) return KSVD_ERR_NULL;
+    *out = nullptr;
+    ksvd_handle* h = new ksvd_handle();
+    h->cfg = *cfg;
+    int rc = check_cfg(h, cfg);
+    if (rc != KSVD_OK) {
+        // keep the handle so the caller can read the message, as documented
+        *out = h;
+        return rc;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        *out = h;
+        return fail(h, KSVD_ERR_HIP, "no HIP device available");
+    }
+    rc = alloc(h, &h->counter, 4, "counter");
+    *out = h;
+    return rc;
+}
+
+int ksvd_destroy(ksvd_handle* h) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!h->allocs.empty()) hipStreamSynchronize(h->stream);
+    for (void* p : h->allocs) hipFree(p);
+    delete h;
+    return KSVD_OK;
+}
+
+int ksvd_set_stream(ksvd_handle* h, void* stream) {
+    if (!h) return KSVD_ERR_NULL;
+    h->stream = static_cast<hipStream_t>(stream);
+    return KSVD_OK;
+}
+
+const char* ksvd_last_error(const ksvd_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int ksvd_sync(ksvd_handle* h) {
+    if (!h) return KSVD_ERR_NULL;
+    return hip_check(h, hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+int ksvd_normcols(ksvd_handle* h, const float* D_in, float* D_out) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D_in || !D_out) return fail(h, KSVD_ERR_NULL, "normcols: null pointer");
+    return hip_check(h, ksvd::launch_normcols(D_in, D_out, h->cfg.n, h->cfg.K, h->stream), "normcols");
+}
+
+int ksvd_at_times_b(ksvd_handle* h, const float* A, int rows, int colsA, const float* B, int colsB, float* C) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!A || !B || !C) return fail(h, KSVD_ERR_NULL, "at_times_b: null pointer");
+    if (rows < 1 || colsA < 1 || colsB < 1)
+        return fail(h, KSVD_ERR_INVALID, "at_times_b: bad shape rows=%d colsA=%d colsB=%d", rows, colsA, colsB);
+    return hip_check(h, ksvd::launch_at_times_b(A, rows, colsA, B, colsB, C, h->cfg.gemm_form, h->stream),
+                     "at_times_b");
+}
+
+int ksvd_gram(ksvd_handle* h, const float* D, float* G) {
+    if (!h) return KSVD_ERR_NULL;
+    return ksvd_at_times_b(h, D, h->cfg.n, h->cfg.K, D, h->cfg.K, G);
+}
+
+int ksvd_correlate(ksvd_handle* h, const float* D, const float* X, float* DtX) {
+    if (!h) return KSVD_ERR_NULL;
+    return ksvd_at_times_b(h, D, h->cfg.n, h->cfg.K, X, h->cfg.N, DtX);
+}
+
+int ksvd_batch_omp(ksvd_handle* h, const float* DtX, const float* G, float* selected_atoms, float* c) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!DtX || !G || !selected_atoms || !c) return fail(h, KSVD_ERR_NULL, "batch_omp: null pointer");
+    ksvd::OmpArgs a = omp_args(h, DtX, G);
+    a.sel = selected_atoms;
+    a.c = c;
+    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp");
+}
+
+int ksvd_batch_omp_compact(ksvd_handle* h, const float* DtX, const float* G, int32_t* idx, float* val,
+                           int32_t* cnt) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!DtX || !G || !idx || !val || !cnt) return fail(h, KSVD_ERR_NULL, "batch_omp_compact: null pointer");
+    ksvd::OmpArgs a = omp_args(h, DtX, G);
+    a.idx = idx;
+    a.val = val;
+    a.cnt = cnt;
+    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp_compact");
+}
+
+int ksvd_batch_omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G, int32_t* idx,
+                                float* val, int32_t* cnt) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D || !X || !G || !idx || !val || !cnt) return fail(h, KSVD_ERR_NULL, "batch_omp_from_signals: null pointer");
+    TRY(alloc(h, &h->dtx, (size_t)h->cfg.K * h->cfg.N, "DtX"));
+    TRY(ksvd_correlate(h, D, X, h->dtx));
+    return ksvd_batch_omp_compact(h, h->dtx, G, idx, val, cnt);
+}
+
+int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const float* xnorm2, const float* eps2,
+                       int32_t* idx, float* val, int32_t* cnt) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!DtX || !G || !xnorm2 || !eps2 || !idx || !val || !cnt)
+        return fail(h, KSVD_ERR_NULL, "batch_omp_eps: null pointer");
+    ksvd::OmpArgs a = omp_args(h, DtX, G);
+    a.idx = idx;
+    a.val = val;
+    a.cnt = cnt;
+    a.xnorm2 = xnorm2;
+    a.eps2 = eps2;
+    a.work_counter = h->counter;
+    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp_eps");
+}
+
+int ksvd_du_init(ksvd_handle* h) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!h->du_ready) return ensure_du(h);  // allocation also resets the state
+    return hip_check(h, ksvd::launch_du_init(h->du, h->stream), "du_init");
+}
+
+int ksvd_dictionary_update(ksvd_handle* h, const float* X, float* D, const int32_t* idx, float* val,
+                           const int32_t* cnt, const uint32_t* perm_host) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!X || !D || !idx || !val || !cnt || !perm_host)
+        return fail(h, KSVD_ERR_NULL, "dictionary_update: null pointer");
+    std::vector<char> seen(h->cfg.K, 0);
+    for (int k = 0; k < h->cfg.K; ++k) {
+        if (perm_host[k] >= (uint32_t)h->cfg.K || seen[perm_host[k]])
+            return fail(h, KSVD_ERR_INVALID, "dictionary_update: perm is not a permutation of 0..K-1");
+        seen[perm_host[k]] = 1;
+    }
+    TRY(ensure_du(h));
+    HIPTRY(h, hipMemcpyAsync(h->du.perm_dev, perm_host, sizeof(uint32_t) * h->cfg.K, hipMemcpyHostToDevice, h->stream),
+           "perm upload");
+    // the host copy of perm may be reused by the caller right after return
+    HIPTRY(h, hipStreamSynchronize(h->stream), "perm upload sync");
+    return hip_check(h, ksvd::launch_du_sweep(h->du, X, D, idx, val, cnt, h->stream), "dictionary_update");
+}
+
+int ksvd_dictionary_update_dense(ksvd_handle* h, const float* X, float* D, float* selected_atoms, const float* c,
+                                 const uint32_t* perm_host) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!selected_atoms || !c) return fail(h, KSVD_ERR_NULL, "dictionary_update_dense: null pointer");
+    const ksvd_config& cf = h->cfg;
+    TRY(alloc(h, &h->cidx, (size_t)cf.N * cf.T, "cidx"));
+    TRY(alloc(h, &h->cval, (size_t)cf.N * cf.T, "cval"));
+    TRY(alloc(h, &h->ccnt, cf.N, "ccnt"));
+    HIPTRY(h, ksvd::launch_dense_to_compact(selected_atoms, c, cf.K, cf.N, cf.T, h->cidx, h->cval, h->ccnt, h->stream),
+           "dense_to_compact");
+    TRY(ksvd_dictionary_update(h, X, D, h->cidx, h->cval, h->ccnt, perm_host));
+    return hip_check(h, ksvd::launch_compact_to_dense(h->cidx, h->cval, h->ccnt, cf.K, cf.N, cf.T, selected_atoms,
+                                                      h->stream),
+                     "compact_to_dense");
+}
+
+int ksvd_error(ksvd_handle* h, const float* X, const float* D, const int32_t* idx, const float* val,
+               const int32_t* cnt, float* err, double* rmse_host) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!X || !D || !idx || !val || !cnt || !err) return fail(h, KSVD_ERR_NULL, "error: null pointer");
+    TRY(ensure_du(h));
+    HIPTRY(h, ksvd::launch_error(h->du, X, D, idx, val, cnt, err, h->stream), "error");
+    if (rmse_host) {
+        HIPTRY(h, hipMemcpyAsync(rmse_host, h->du.rmse_dev, sizeof(double), hipMemcpyDeviceToHost, h->stream),
+               "rmse download");
+        HIPTRY(h, hipStreamSynchronize(h->stream), "rmse sync");
+    }
+    return KSVD_OK;
+}
+
+int ksvd_clear_dict(ksvd_handle* h, const float* X, float* D, const int32_t* idx, const float* val,
+                    const int32_t* cnt, float* err) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!X || !D || !idx || !val || !cnt || !err) return fail(h, KSVD_ERR_NULL, "clear_dict: null pointer");
+    TRY(ensure_du(h));
+    return hip_check(h, ksvd::launch_clear_dict(h->du, X, D, idx, val, cnt, err, h->stream), "clear_dict");
+}
+
+int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, float* D_out, int32_t* idx, float* val,
+                            int32_t* cnt, int iterations, double* rmse_host) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!X || !D_in || !D_out || !idx || !val || !cnt)
+        return fail(h, KSVD_ERR_NULL, "execute_iterations: null pointer");
+    if (iterations < 0) return fail(h, KSVD_ERR_INVALID, "iterations=%d", iterations);
+    const ksvd_config& c = h->cfg;
+    TRY(ensure_du(h));
+    TRY(alloc(h, &h->dtx, (size_t)c.K * c.N, "DtX"));
+    TRY(alloc(h, &h->gram, (size_t)c.K * c.K, "G"));
+    TRY(alloc(h, &h->errv, c.N, "err"));
+    if (iterations > h->rmse_cap || iterations > h->perm_cap) {
+        // grow: the previous buffers stay owned by the handle until destroy
+        h->rmse_all = nullptr;
+        h->perm_all = nullptr;
+        TRY(alloc(h, &h->rmse_all, iterations > 0 ? iterations : 1, "rmse_all"));
+        TRY(alloc(h, &h->perm_all, (size_t)(iterations > 0 ? iterations : 1) * c.K, "perm_all"));
+        h->rmse_cap = h->perm_cap = iterations;
+    }
+    // all permutations up front: the loop below never waits on the host
+    std::vector<uint32_t> perms((size_t)(iterations > 0 ? iterations : 1) * c.K);
+    TRY(ksvd_perm_sequence(c.perm_seed, c.K, iterations, perms.data()));
+    if (iterations > 0)
+        HIPTRY(h, hipMemcpyAsync(h->perm_all, perms.data(), sizeof(uint32_t) * perms.size(), hipMemcpyHostToDevice,
+                                 h->stream),
+               "perm upload");
+    HIPTRY(h, hipStreamSynchronize(h->stream), "perm upload sync");  // `perms` is pageable and local
+    HIPTRY(h, ksvd::launch_normcols(D_in, D_out, c.n, c.K, h->stream), "normcols");
+    double* rmse_slot = h->du.rmse_dev;
+    for (int it = 0; it < iterations; ++it) {
+        HIPTRY(h, ksvd::launch_at_times_b(D_out, c.n, c.K, D_out, c.K, h->gram, c.gemm_form, h->stream), "gram");
+        HIPTRY(h, ksvd::launch_at_times_b(D_out, c.n, c.K, X, c.N, h->dtx, c.gemm_form, h->stream), "correlate");
+        ksvd::OmpArgs a = omp_args(h, h->dtx, h->gram);
+        a.idx = idx;
+        a.val = val;
+        a.cnt = cnt;
+        HIPTRY(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp");
+        HIPTRY(h, ksvd::launch_du_init(h->du, h->stream), "du_init");
+        h->du.perm_dev = h->perm_all + (size_t)it * c.K;
+        HIPTRY(h, ksvd::launch_du_sweep(h->du, X, D_out, idx, val, cnt, h->stream), "dictionary_update");
+        h->du.rmse_dev = h->rmse_all + it;
+        HIPTRY(h, ksvd::launch_error(h->du, X, D_out, idx, val, cnt, h->errv, h->stream), "error");
+        HIPTRY(h, ksvd::launch_clear_dict(h->du, X, D_out, idx, val, cnt, h->errv, h->stream), "clear_dict");
+    }
+    h->du.rmse_dev = rmse_slot;
+    if (rmse_host && iterations > 0) {
+        HIPTRY(h, hipMemcpyAsync(rmse_host, h->rmse_all, sizeof(double) * iterations, hipMemcpyDeviceToHost, h->stream),
+               "rmse download");
+        HIPTRY(h, hipStreamSynchronize(h->stream), "rmse sync");
+    }
+    return KSVD_OK;
+}
+
+int ksvd_perm_sequence(uint64_t seed, int K, int iterations, uint32_t* out) {
+    if (!out) return KSVD_ERR_NULL;
+    if (K < 1 || iterations < 0) return KSVD_ERR_INVALID;
+    std::vector<uint32_t> buf(K);
+    for (int k = 0; k < K; ++k) buf[k] = (uint32_t)k;
+    uint64_t st = seed;
+    for (int it = 0; it < iterations; ++it) {
+        shuffle(buf.data(), K, &st);
+        std::memcpy(out + (size_t)it * K, buf.data(), sizeof(uint32_t) * K);
+    }
+    return KSVD_OK;
+}
+
+int ksvd_omp_adjoint(ksvd_handle* h, const float* D, const float* X, const int32_t* idx, const float* val,
+                     const int32_t* cnt, const float* gval, float* gD, float* gX) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D || !X || !idx || !val || !cnt || !gval || !gD || !gX)
+        return fail(h, KSVD_ERR_NULL, "omp_adjoint: null pointer");
+    const ksvd_config& c = h->cfg;
+    return hip_check(h, ksvd::launch_omp_adjoint(D, X, idx, val, cnt, gval, gD, gX, c.n, c.K, c.N, c.T, h->stream),
+                     "omp_adjoint");
+}
+
+}  // extern "C"

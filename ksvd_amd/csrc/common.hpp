@@ -1,0 +1,163 @@
+// common.hpp -- shared device helpers and internal launcher declarations for
+// the MI355X K-SVD engine.  gfx950 (CDNA4) only: wave64, MFMA, 160 KB LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ksvd {
+
+constexpr int WAVE = 64;
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+// ---------------------------------------------------------------------------
+// Wave64 cross-lane reductions.  The reference's warp32 __shfl_down trees
+// (S/ReductionBase.cuh:40-136) become butterfly reductions over 64 lanes
+// (ds_swizzle / DPP via __shfl_xor); every lane ends with the result.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+// Arg-max with the reference's tie rule: larger value wins, equal values keep
+// the lower index (S/OMP.cu:201-213 scans with strict '>';
+// S/ReductionBase.cuh:40-51 keeps its own lane on ties).
+__device__ __forceinline__ void wave_argmax(float& v, int& i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float ov = __shfl_xor(v, o, WAVE);
+        int oi = __shfl_xor(i, o, WAVE);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+
+__device__ __forceinline__ float readlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double readlane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Wave-uniform copies (scalar registers) of values every lane already agrees
+// on, so that control flow on them compiles to scalar branches.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) { return readlane(v, 0); }
+
+// Block-wide sum for blockDim.x <= 1024 (any multiple of 64); result valid in
+// every thread.  `red` must hold 16 entries.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    __syncthreads();
+    if (lane_id() == 0) red[w] = v;
+    __syncthreads();
+    T s = 0;
+    for (int i = 0; i < nw; ++i) s += red[i];  // fixed order: deterministic
+    return s;
+}
+
+}  // namespace ksvd
+
+// ---------------------------------------------------------------------------
+// Internal launchers (implemented in the .hip files, called by capi.cpp).
+// All enqueue on `st`; return a hipError_t.
+// ---------------------------------------------------------------------------
+namespace ksvd {
+
+// C (colsA x colsB) = A^T B, column-major, lda = ldb = rows.  form: 0 fp32,
+// 1 bf16, 2 bf16x3.
+hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B, long colsB, float* C,
+                             int form, hipStream_t st);
+hipError_t launch_normcols(const float* Din, float* Dout, int n, int K, hipStream_t st);
+
+struct OmpArgs {
+    const float* DtX;   // K x N
+    const float* G;     // K x K
+    int K, N, T;
+    float* sel;         // optional K x N slot map
+    float* c;           // optional T x N coefficients
+    int32_t* idx;       // optional T x N compact
+    float* val;         // optional T x N compact
+    int32_t* cnt;       // optional N
+    const float* xnorm2;  // error mode (else NULL)
+    const float* eps2;
+    int32_t* work_counter;  // persistent work-stealing counter (error mode)
+    float pivot_fp64;
+};
+hipError_t launch_batch_omp(const OmpArgs& a, hipStream_t st);
+
+// Dictionary update / error / clear (dict_update.hip)
+struct DuWork {
+    int n, K, N, T, max_signals, use_thresh;
+    double mu_thresh;
+    // scratch (device)
+    int32_t* count;      // K
+    int32_t* ptr;        // K + 1
+    int32_t* chunk_hist; // K x nchunks
+    int32_t* list_pos;   // N*T
+    float* gn2;          // K
+    float* Y;            // n x K
+    float* vpart;        // vparts x K
+    float* avec;         // n
+    float* uvec;         // K
+    float* scal;         // 8 scalars (norm etc.)
+    float* err_cand;     // max_signals
+    int32_t* unused;     // N
+    int32_t* us_counter; // 1
+    int32_t* bitmap;     // N
+    int32_t* replaced;   // K
+    uint32_t* perm_dev;  // K
+    int32_t* usecount;   // K
+    float* Gtmp;         // K x K (clearDict Gram)
+    double* rmse_part;   // nparts
+    double* rmse_dev;    // 1
+    int32_t* repl_list;  // K (clearDict replaced-in-sweep list) + 1 count
+    int nchunks, chunk;
+    int vparts;
+    int rmse_parts;
+};
+hipError_t launch_du_init(DuWork& w, hipStream_t st);
+hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* cnt, hipStream_t st);
+hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* idx, float* val,
+                           const int32_t* cnt, hipStream_t st);
+hipError_t launch_error(DuWork& w, const float* X, const float* D, const int32_t* idx, const float* val,
+                        const int32_t* cnt, float* err, hipStream_t st);
+hipError_t launch_clear_dict(DuWork& w, const float* X, float* D, const int32_t* idx, const float* val,
+                             const int32_t* cnt, float* err, hipStream_t st);
+hipError_t launch_dense_to_compact(const float* sel, const float* c, int K, int N, int T, int32_t* idx,
+                                   float* val, int32_t* cnt, hipStream_t st);
+hipError_t launch_compact_to_dense(const int32_t* idx, const float* val, const int32_t* cnt, int K, int N,
+                                   int T, float* gamma_dense, hipStream_t st);
+
+// Adjoint (adjoint.hip)
+hipError_t launch_omp_adjoint(const float* D, const float* X, const int32_t* idx, const float* val,
+                              const int32_t* cnt, const float* gval, float* gD, float* gX, int n, int K,
+                              int N, int T, hipStream_t st);
+
+}  // namespace ksvd

@@ -1,0 +1,681 @@
+// dict_update.hip -- approximate K-SVD dictionary update, error and
+// dictionary clearing on CDNA4.
+//
+// Reference: DictionaryUpdate::updateDictionary (S/DictionaryUpdate.cu:45-203)
+// launches ~12 kernels per atom including a full n x N x K cuBLAS GEMM for the
+// unused-atom test, and builds its per-atom lists with atomicAdd slots
+// (nondeterministic order).  This engine works on the compact Gamma
+// (idx/val/cnt), builds the per-atom lists once per sweep in ascending signal
+// order with a two-level counting sort (no order-dependent atomics), and runs
+// five small launches per atom whose work is proportional to the atom's list
+// length N_p (not N or K x N):
+//   1. special-case error of the first min(N, max_signals) unused signals
+//      (only when N_p = 0; sparse reconstruction, no GEMM)
+//   2. special-case arg-max + replacement
+//   3. v = Gamma(:, I_p) gamma_p^T  -- per-block LDS accumulation
+//   4. a = Y_p - D v + d_p ||gamma_p||^2, ||a||, u = D^T a  -- one workgroup
+//   5. gamma_new = (a^T x - u^T Gamma(:,s) + u_p gamma)/||a||, d_p = a/||a||
+// All data-dependent branching is device-side (early exits on N_p), so the
+// host loop never synchronises and the sequence is graph-capturable.
+#include "common.hpp"
+
+namespace ksvd {
+
+namespace {
+
+constexpr int UPD_BLOCKS = 2048;  // grid of the per-entry update kernel (grid-stride)
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- state reset: S/KSVD.cu:442-457 ---------------------------------------
+__global__ void du_init_kernel(int32_t* unused, int32_t* usc, int32_t* bitmap, int32_t* replaced, int N, int K) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) { unused[i] = (int32_t)i; bitmap[i] = 1; }
+    if (i < K) replaced[i] = 0;
+    if (i == 0) *usc = 0;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, long n, int32_t v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_f32_kernel(float* p, long n, float v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- per-atom lists (replaces sprow_kernel, S/DictionaryUpdate.cu:378-395) --
+// chunk_hist[c*K + k] = number of entries of atom k among signals of chunk c.
+__global__ __launch_bounds__(256) void du_hist_kernel(const int32_t* __restrict__ idx, const int32_t* __restrict__ cnt,
+                                                      int N, int T, int K, int CH, int32_t* __restrict__ chunk_hist) {
+    extern __shared__ int32_t h[];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    const long s0 = (long)blockIdx.x * CH;
+    const long s1 = min((long)N, s0 + CH);
+    for (long s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+        const int c = cnt[s];
+        for (int j = 0; j < c; ++j) atomicAdd(&h[idx[s * T + j]], 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) chunk_hist[(long)blockIdx.x * K + k] = h[k];
+}
+
+// Per atom: exclusive scan of chunk_hist over chunks (in place) and the total.
+__global__ __launch_bounds__(256) void du_scan_chunks_kernel(int32_t* __restrict__ chunk_hist, int nch, int K,
+                                                             int32_t* __restrict__ count) {
+    __shared__ int32_t buf[256];
+    const int k = blockIdx.x;
+    int32_t carry = 0;
+    for (int c0 = 0; c0 < nch; c0 += 256) {
+        const int c = c0 + threadIdx.x;
+        const int32_t v = (c < nch) ? chunk_hist[(long)c * K + k] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {  // Hillis-Steele inclusive scan
+            const int32_t t = (threadIdx.x >= o) ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (c < nch) chunk_hist[(long)c * K + k] = carry + buf[threadIdx.x] - v;
+        carry += buf[255];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) count[k] = carry;
+}
+
+// ptr = exclusive scan of count (single block).
+__global__ __launch_bounds__(1024) void du_scan_ptr_kernel(const int32_t* __restrict__ count, int K,
+                                                           int32_t* __restrict__ ptr) {
+    __shared__ int32_t buf[1024];
+    int32_t carry = 0;
+    for (int k0 = 0; k0 < K; k0 += 1024) {
+        const int k = k0 + threadIdx.x;
+        const int32_t v = (k < K) ? count[k] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int32_t t = (threadIdx.x >= o) ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (k < K) ptr[k] = carry + buf[threadIdx.x] - v;
+        carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ptr[K] = carry;
+}
+
+// One wave per chunk: signals in ascending order, lane j = slot j; the atoms of
+// one signal are distinct, so the LDS cursors never collide within a step.
+__global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__ idx, const int32_t* __restrict__ cnt,
+                                                     int N, int T, int K, int CH,
+                                                     const int32_t* __restrict__ chunk_off,
+                                                     const int32_t* __restrict__ ptr,
+                                                     int32_t* __restrict__ list_pos) {
+    extern __shared__ int32_t cur[];
+    const int lane = threadIdx.x;
+    const long c = blockIdx.x;
+    for (int k = lane; k < K; k += 64) cur[k] = ptr[k] + chunk_off[c * K + k];
+    wave_lds_fence();
+    const long s0 = c * CH;
+    const long s1 = min((long)N, s0 + CH);
+    // software pipeline: fetch the next signal's slot while placing this one
+    int nc = (s0 < s1) ? cnt[s0] : 0;
+    int nk = (s0 < s1 && lane < nc) ? idx[s0 * T + lane] : 0;
+    for (long s = s0; s < s1; ++s) {
+        const int cc = nc, kk = nk;
+        if (s + 1 < s1) {
+            nc = cnt[s + 1];
+            nk = (lane < nc) ? idx[(s + 1) * T + lane] : 0;
+        }
+        if (lane < cc) {
+            const int p = cur[kk];
+            cur[kk] = p + 1;
+            list_pos[p] = (int32_t)(s * T + lane);
+        }
+        wave_lds_fence();
+    }
+}
+
+// ---- precompute ||gamma_k||^2 and Y_k = X(:, I_k) gamma_k -----------------
+// S/ReductionKernels.cu:31-98 (STAGE1 + STAGE2_together_collincomb)
+__global__ __launch_bounds__(256) void du_prep_kernel(const float* __restrict__ X, const float* __restrict__ val,
+                                                      const int32_t* __restrict__ list_pos,
+                                                      const int32_t* __restrict__ ptr, int n, int T,
+                                                      float* __restrict__ gn2, float* __restrict__ Y) {
+    __shared__ float red[1024];
+    __shared__ float wred[16];
+    const int k = blockIdx.x;
+    const int beg = ptr[k], end = ptr[k + 1];
+    float sq = 0.f;
+    for (int m = beg + threadIdx.x; m < end; m += blockDim.x) {
+        const float v = val[list_pos[m]];
+        sq += v * v;
+    }
+    sq = block_sum(sq, wred);
+    if (threadIdx.x == 0) gn2[k] = sq;
+
+    const int npad = min(256, ((n + 63) / 64) * 64);
+    const int ng = 256 / npad;
+    const int g = threadIdx.x / npad, tr = threadIdx.x % npad;
+    for (int t0 = 0; t0 < n; t0 += npad) {
+        const int t = t0 + tr;
+        float acc = 0.f;
+        if (t < n) {
+            int m = beg + g;
+            for (; m + 3 * ng < end; m += 4 * ng) {
+                const int p0 = list_pos[m], p1 = list_pos[m + ng], p2 = list_pos[m + 2 * ng],
+                          p3 = list_pos[m + 3 * ng];
+                const float x0 = X[(long)(p0 / T) * n + t], x1 = X[(long)(p1 / T) * n + t];
+                const float x2 = X[(long)(p2 / T) * n + t], x3 = X[(long)(p3 / T) * n + t];
+                acc += x0 * val[p0];
+                acc += x1 * val[p1];
+                acc += x2 * val[p2];
+                acc += x3 * val[p3];
+            }
+            for (; m < end; m += ng) {
+                const int p0 = list_pos[m];
+                acc += X[(long)(p0 / T) * n + t] * val[p0];
+            }
+        }
+        __syncthreads();
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (g == 0 && t < n) {
+            float s = 0.f;
+            for (int q = 0; q < ng; ++q) s += red[q * npad + tr];
+            Y[(long)k * n + t] = s;
+        }
+    }
+}
+
+// ---- per-atom step kernels -------------------------------------------------
+// Sparse reconstruction error of one signal by one wave: ||x - D gamma_s||^2.
+__device__ __forceinline__ float wave_signal_error(const float* __restrict__ X, const float* __restrict__ D,
+                                                   const int32_t* __restrict__ idx,
+                                                   const float* __restrict__ val, int c, long s, int n, int T,
+                                                   int lane) {
+    const int il = (lane < c) ? idx[s * T + lane] : 0;
+    const float vl = (lane < c) ? val[s * T + lane] : 0.f;
+    float e = 0.f;
+    for (int t0 = 0; t0 < n; t0 += 64) {
+        const int t = t0 + lane;
+        float xh = 0.f;
+        for (int j = 0; j < c; ++j) {
+            const int col = readlane(il, j);
+            const float g = readlane(vl, j);
+            if (t < n) xh += D[(long)col * n + t] * g;
+        }
+        if (t < n) {
+            const float d = X[s * n + t] - xh;
+            e += d * d;
+        }
+    }
+    return wave_sum(e);
+}
+
+// S/ReductionKernels.cu:436-456 (SCase_err_stage1), without the dense D*Gamma.
+__global__ __launch_bounds__(256) void du_special_err_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                             const int32_t* __restrict__ count,
+                                                             const float* __restrict__ X,
+                                                             const float* __restrict__ D,
+                                                             const int32_t* __restrict__ idx,
+                                                             const float* __restrict__ val,
+                                                             const int32_t* __restrict__ cnt,
+                                                             const int32_t* __restrict__ unused,
+                                                             const int32_t* __restrict__ usc, int n, int N, int T,
+                                                             int M, float* __restrict__ err_cand) {
+    const int p = (int)perm[jj];
+    if (count[p] != 0) return;
+    const int live = N - *usc;
+    const int lim = min(M, live);
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= lim) return;
+    const int lane = threadIdx.x & 63;
+    const long sig = unused[b];
+    const float e = wave_signal_error(X, D, idx, val, cnt[sig], sig, n, T, lane);
+    if (lane == 0) err_cand[b] = e;
+}
+
+// S/ReductionKernels.cu:462-539 + SCupdate S/DictionaryUpdate.cu:406-427.
+__global__ __launch_bounds__(1024) void du_special_apply_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                                const int32_t* __restrict__ count,
+                                                                const float* __restrict__ X, float* __restrict__ D,
+                                                                int32_t* __restrict__ unused,
+                                                                int32_t* __restrict__ usc,
+                                                                int32_t* __restrict__ bitmap,
+                                                                int32_t* __restrict__ replaced, int n, int N, int M,
+                                                                const float* __restrict__ err_cand) {
+    __shared__ float rv[16];
+    __shared__ int ri[16];
+    __shared__ float wred[16];
+    const int p = (int)perm[jj];
+    if (count[p] != 0) return;
+    const int counter = *usc;
+    const int lim = min(M, N - counter);
+    float bv = 0.f;  // reference seeds lanes with value 0 (S/ReductionKernels.cu:468-476)
+    int bi = 0;
+    for (int b = threadIdx.x; b < lim; b += blockDim.x) {
+        const float v = err_cand[b];
+        if (v > bv) { bv = v; bi = b; }
+    }
+    wave_argmax(bv, bi);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { rv[w] = bv; ri[w] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = rv[0];
+        int i = ri[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+            if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
+        ri[0] = i;
+    }
+    __syncthreads();
+    const int best = ri[0];
+    const long sig = unused[best];
+    float sq = 0.f;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const float x = X[sig * n + t];
+        sq += x * x;
+    }
+    sq = block_sum(sq, wred);
+    const float nrm = sqrtf(sq);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unused[best] = unused[N - counter - 1];
+        *usc = counter + 1;
+        replaced[p] = 1;
+        bitmap[sig] = 0;
+    }
+}
+
+// v = Gamma(:, I_p) gamma_p^T over rows != p  (S/ReductionKernels.cu:105-139);
+// one thread per list entry, LDS float atomics, one partial vector per block.
+__global__ __launch_bounds__(256) void du_v_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                   const int32_t* __restrict__ count,
+                                                   const int32_t* __restrict__ ptr,
+                                                   const int32_t* __restrict__ list_pos,
+                                                   const int32_t* __restrict__ idx, const float* __restrict__ val,
+                                                   const int32_t* __restrict__ cnt, int K, int T, int vparts,
+                                                   float* __restrict__ vpart) {
+    extern __shared__ float v[];
+    const int p = (int)perm[jj];
+    const int Np = count[p];
+    if (Np == 0) return;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) v[k] = 0.f;
+    __syncthreads();
+    const int beg = ptr[p];
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < Np; m += vparts * blockDim.x) {
+        const int pos = list_pos[beg + m];
+        const long s = pos / T;
+        const float gp = val[pos];
+        const int c = cnt[s];
+        const int32_t* ip = idx + s * T;
+        const float* vp = val + s * T;
+        for (int j = 0; j < c; ++j) {
+            const int r = ip[j];
+            if (r != p) atomicAdd(&v[r], vp[j] * gp);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) vpart[(long)blockIdx.x * K + k] = v[k];
+}
+
+// a = Y_p - D v + d_p ||gamma_p||^2   (device_A_times_x_modified, S/Multiplication.cu:177-193)
+// ||a|| and u = D^T a                  (rowlincomb, S/ReductionKernels.cu:196-250)
+__global__ __launch_bounds__(1024) void du_atom_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                       const int32_t* __restrict__ count,
+                                                       const float* __restrict__ vpart, int vparts,
+                                                       const float* __restrict__ gn2, const float* __restrict__ Y,
+                                                       const float* __restrict__ D, int n, int K,
+                                                       float* __restrict__ avec, float* __restrict__ uvec,
+                                                       float* __restrict__ scal) {
+    extern __shared__ float sh[];
+    float* v = sh;          // K
+    float* a = sh + K;      // n
+    float* red = a + n;     // 1024
+    __shared__ float wred[16];
+    const int p = (int)perm[jj];
+    if (count[p] == 0) return;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        float s = 0.f;
+        for (int b = 0; b < vparts; ++b) s += vpart[(long)b * K + k];
+        v[k] = s;
+    }
+    __syncthreads();
+    const float g2 = gn2[p];
+    if (threadIdx.x == 0) v[p] = g2;
+    __syncthreads();
+    // D v: thread (g, tr) sums columns r = g, g+ng, ... for row t
+    const int npad = min(1024, ((n + 63) / 64) * 64);
+    const int ng = 1024 / npad;
+    const int g = threadIdx.x / npad, tr = threadIdx.x % npad;
+    for (int t0 = 0; t0 < n; t0 += npad) {
+        const int t = t0 + tr;
+        float acc = 0.f;
+        if (t < n)
+            for (int r = g; r < K; r += ng) acc += D[(long)r * n + t] * v[r];
+        __syncthreads();
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        if (g == 0 && t < n) {
+            float s = 0.f;
+            for (int q = 0; q < ng; ++q) s += red[q * npad + tr];
+            a[t] = Y[(long)p * n + t] - s + D[(long)p * n + t] * g2;
+        }
+    }
+    __syncthreads();
+    float sq = 0.f;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        sq += a[t] * a[t];
+        avec[t] = a[t];
+    }
+    sq = block_sum(sq, wred);
+    if (threadIdx.x == 0) scal[0] = sqrtf(sq);
+    // u = D^T a: one wave per column
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int r = w; r < K; r += nw) {
+        float s = 0.f;
+        for (int t = lane; t < n; t += 64) s += D[(long)r * n + t] * a[t];
+        s = wave_sum(s);
+        if (lane == 0) uvec[r] = s;
+    }
+}
+
+// gamma_new (device_normal_x_times_A_modified, S/Multiplication.cu:198-225):
+// one wave per list entry; the slots of signal s are held one per lane.
+__global__ __launch_bounds__(256) void du_update_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                        const int32_t* __restrict__ count,
+                                                        const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ list_pos,
+                                                        const int32_t* __restrict__ idx, float* __restrict__ val,
+                                                        const int32_t* __restrict__ cnt,
+                                                        const float* __restrict__ X, float* __restrict__ D,
+                                                        const float* __restrict__ avec,
+                                                        const float* __restrict__ uvec,
+                                                        const float* __restrict__ scal, int n, int T) {
+    extern __shared__ float a[];
+    const int p = (int)perm[jj];
+    const int Np = count[p];
+    if (Np == 0) return;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) a[t] = avec[t];
+    __syncthreads();
+    const float nrm = scal[0];
+    const float up = uvec[p];
+    const int lane = threadIdx.x & 63;
+    const int beg = ptr[p];
+    const int wg = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwg = gridDim.x * 4;
+    for (int m = wg; m < Np; m += nwg) {
+        const int pos = list_pos[beg + m];
+        const long s = pos / T;
+        const int c = cnt[s];
+        float wdot = 0.f;
+        for (int t = lane; t < n; t += 64) wdot += X[s * n + t] * a[t];
+        const float su = (lane < c) ? val[s * T + lane] * uvec[idx[s * T + lane]] : 0.f;
+        wdot = wave_sum(wdot);
+        const float sum = wave_sum(su);
+        if (lane == 0) {
+            const float gold = val[pos];
+            val[pos] = (wdot - sum + up * gold) / nrm;
+        }
+    }
+    if (blockIdx.x == 0)
+        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = a[t] / nrm;
+}
+
+// ---- error: S/KSVD.cu:235-262, S/ReductionKernels.cu:257-322 ---------------
+constexpr int ERR_BLOCKS = 2048;
+
+__global__ __launch_bounds__(256) void error_kernel(const float* __restrict__ X, const float* __restrict__ D,
+                                                    const int32_t* __restrict__ idx, const float* __restrict__ val,
+                                                    const int32_t* __restrict__ cnt,
+                                                    const int32_t* __restrict__ bitmap, int n, int N, int T,
+                                                    float* __restrict__ err, double* __restrict__ part) {
+    __shared__ double wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double acc = 0.0;
+    for (long s = (long)blockIdx.x * 4 + w; s < N; s += (long)gridDim.x * 4) {
+        const float e = wave_signal_error(X, D, idx, val, cnt[s], s, n, T, lane);
+        if (lane == 0) err[s] = e * (float)bitmap[s];
+        acc += (double)e;
+    }
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
+__global__ __launch_bounds__(1024) void error_finish_kernel(const double* __restrict__ part, int nparts, double denom,
+                                                            double* __restrict__ rmse) {
+    __shared__ double wred[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
+    s = block_sum(s, wred);
+    if (threadIdx.x == 0) *rmse = sqrt(s / denom);
+}
+
+// ---- clearDict: S/KSVD.cu:138-226 ------------------------------------------
+// usecount_k = #{s : gamma_ks^2 > 1e-7}  (use_count_kernel, S/KSVD.cu:403-415)
+__global__ void usecount_kernel(const int32_t* __restrict__ idx, const float* __restrict__ val,
+                                const int32_t* __restrict__ cnt, int N, int T, int32_t* __restrict__ usecount) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    const int c = cnt[s];
+    for (int j = 0; j < c; ++j) {
+        const float v = val[s * T + j];
+        if (v * v > 1e-7f) atomicAdd(&usecount[idx[s * T + j]], 1);
+    }
+}
+
+// Sequential sweep over atoms in one workgroup.  G holds D^T D for the D at
+// entry; when atom j is replaced, row and column j of G are recomputed so that
+// later steps see the current dictionary (the reference recomputes
+// G_j = D^T d_j with a GEMV per atom, S/KSVD.cu:164-173).
+__global__ __launch_bounds__(1024) void clear_sweep_kernel(const float* __restrict__ X, float* __restrict__ D,
+                                                           float* __restrict__ G,
+                                                           const int32_t* __restrict__ usecount,
+                                                           const int32_t* __restrict__ replaced,
+                                                           float* __restrict__ err, int n, int K, int N,
+                                                           double mu2, int use_thresh) {
+    __shared__ float rv[16];
+    __shared__ int ri[16];
+    __shared__ float wred[16];
+    __shared__ int flag_sh;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int j = 0; j < K; ++j) {
+        // max_{i != j} G_j[i]^2   (device_max, S/ReductionKernels.cu:393-408)
+        float mx = 0.f;
+        for (int i = threadIdx.x; i < K; i += blockDim.x) {
+            const float g = G[(long)j * K + i];
+            if (i != j) mx = fmaxf(mx, g * g);
+        }
+        mx = wave_max(mx);
+        if (lane == 0) rv[w] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float m = rv[0];
+            for (int q = 1; q < nw; ++q) m = fmaxf(m, rv[q]);
+            flag_sh = (((double)m > mu2) || (usecount[j] < use_thresh)) && (replaced[j] == 0);
+        }
+        __syncthreads();
+        if (!flag_sh) continue;
+        // arg-max of err (S/ReductionKernels.cu:329-386), lowest index on ties
+        float bv = -1.f;
+        int bi = 0;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+            const float e = err[i];
+            if (e > bv) { bv = e; bi = i; }
+        }
+        wave_argmax(bv, bi);
+        if (lane == 0) { rv[w] = bv; ri[w] = bi; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float v = rv[0];
+            int i = ri[0];
+            for (int q = 1; q < nw; ++q)
+                if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
+            ri[0] = i;
+            err[i] = 0.f;
+        }
+        __syncthreads();
+        const long best = ri[0];
+        float sq = 0.f;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const float x = X[best * n + t];
+            sq += x * x;
+        }
+        sq = block_sum(sq, wred);
+        const float nrm = sqrtf(sq);  // EUnorm S/ReductionKernels.cu:413-430
+        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)j * n + t] = X[best * n + t] / nrm;
+        __threadfence_block();
+        __syncthreads();
+        // refresh row/column j of G for the new atom
+        for (int i = w; i < K; i += nw) {
+            float s = 0.f;
+            for (int t = lane; t < n; t += 64) s += D[(long)i * n + t] * D[(long)j * n + t];
+            s = wave_sum(s);
+            if (lane == 0) { G[(long)j * K + i] = s; G[(long)i * K + j] = s; }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// ---- layout conversions ------------------------------------------------------
+// Slot map + c -> compact (every nonzero slot, as sprow_kernel reads them).
+__global__ __launch_bounds__(256) void dense_to_compact_kernel(const float* __restrict__ sel,
+                                                               const float* __restrict__ c, int K, int N, int T,
+                                                               int32_t* __restrict__ idx, float* __restrict__ val,
+                                                               int32_t* __restrict__ cnt) {
+    const long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= N) return;
+    const int lane = threadIdx.x & 63;
+    int mxs = 0;
+    for (int k = lane; k < K; k += 64) {
+        const int slot = (int)sel[s * K + k];
+        if (slot > 0 && slot <= T) {
+            idx[s * T + slot - 1] = k;
+            val[s * T + slot - 1] = c[s * T + slot - 1];
+            mxs = max(mxs, slot);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) mxs = max(mxs, __shfl_xor(mxs, o, 64));
+    if (lane >= mxs && lane < T) { idx[s * T + lane] = -1; val[s * T + lane] = 0.f; }
+    if (lane == 0) cnt[s] = mxs;
+}
+
+// compact -> dense Gamma values (K x N); each entry written once.
+__global__ __launch_bounds__(256) void compact_to_dense_kernel(const int32_t* __restrict__ idx,
+                                                               const float* __restrict__ val,
+                                                               const int32_t* __restrict__ cnt, int K, int N, int T,
+                                                               float* __restrict__ out) {
+    const long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= N) return;
+    const int lane = threadIdx.x & 63;
+    const int c = cnt[s];
+    const int il = (lane < c) ? idx[s * T + lane] : -1;
+    const float vl = (lane < c) ? val[s * T + lane] : 0.f;
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        const int k = k0 + lane;
+        float v = 0.f;
+        for (int j = 0; j < c; ++j)
+            if (readlane(il, j) == k) v = readlane(vl, j);
+        if (k < K) out[s * K + k] = v;
+    }
+}
+
+}  // namespace
+
+// ---- launchers ---------------------------------------------------------------
+static inline unsigned blocks_for(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_du_init(DuWork& w, hipStream_t st) {
+    const long m = w.N > w.K ? w.N : w.K;
+    hipLaunchKernelGGL(du_init_kernel, dim3(blocks_for(m, 256)), dim3(256), 0, st, w.unused, w.us_counter, w.bitmap,
+                       w.replaced, w.N, w.K);
+    return hipGetLastError();
+}
+
+hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* cnt, hipStream_t st) {
+    hipLaunchKernelGGL(du_hist_kernel, dim3(w.nchunks), dim3(256), (size_t)w.K * 4, st, idx, cnt, w.N, w.T, w.K,
+                       w.chunk, w.chunk_hist);
+    hipLaunchKernelGGL(du_scan_chunks_kernel, dim3(w.K), dim3(256), 0, st, w.chunk_hist, w.nchunks, w.K, w.count);
+    hipLaunchKernelGGL(du_scan_ptr_kernel, dim3(1), dim3(1024), 0, st, w.count, w.K, w.ptr);
+    hipLaunchKernelGGL(du_fill_kernel, dim3(w.nchunks), dim3(64), (size_t)w.K * 4, st, idx, cnt, w.N, w.T, w.K,
+                       w.chunk, w.chunk_hist, w.ptr, w.list_pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* idx, float* val, const int32_t* cnt,
+                           hipStream_t st) {
+    hipError_t e = launch_du_build_lists(w, idx, cnt, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(du_prep_kernel, dim3(w.K), dim3(256), 0, st, X, val, w.list_pos, w.ptr, w.n, w.T, w.gn2, w.Y);
+    const int M = w.N < w.max_signals ? w.N : w.max_signals;
+    const unsigned spec_blocks = blocks_for(M, 4);
+    const size_t atom_lds = ((size_t)w.K + w.n + 1024) * 4;
+    for (int jj = 0; jj < w.K; ++jj) {
+        hipLaunchKernelGGL(du_special_err_kernel, dim3(spec_blocks), dim3(256), 0, st, jj, w.perm_dev, w.count, X, D,
+                           idx, val, cnt, w.unused, w.us_counter, w.n, w.N, w.T, M, w.err_cand);
+        hipLaunchKernelGGL(du_special_apply_kernel, dim3(1), dim3(1024), 0, st, jj, w.perm_dev, w.count, X, D,
+                           w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, M, w.err_cand);
+        hipLaunchKernelGGL(du_v_kernel, dim3(w.vparts), dim3(256), (size_t)w.K * 4, st, jj, w.perm_dev, w.count,
+                           w.ptr, w.list_pos, idx, val, cnt, w.K, w.T, w.vparts, w.vpart);
+        hipLaunchKernelGGL(du_atom_kernel, dim3(1), dim3(1024), atom_lds, st, jj, w.perm_dev, w.count, w.vpart,
+                           w.vparts, w.gn2, w.Y, D, w.n, w.K, w.avec, w.uvec, w.scal);
+        hipLaunchKernelGGL(du_update_kernel, dim3(UPD_BLOCKS), dim3(256), (size_t)w.n * 4, st, jj, w.perm_dev,
+                           w.count, w.ptr, w.list_pos, idx, val, cnt, X, D, w.avec, w.uvec, w.scal, w.n, w.T);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_error(DuWork& w, const float* X, const float* D, const int32_t* idx, const float* val,
+                        const int32_t* cnt, float* err, hipStream_t st) {
+    hipLaunchKernelGGL(error_kernel, dim3(ERR_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.bitmap, w.n, w.N, w.T,
+                       err, w.rmse_part);
+    hipLaunchKernelGGL(error_finish_kernel, dim3(1), dim3(1024), 0, st, w.rmse_part, ERR_BLOCKS,
+                       (double)w.n * (double)w.N, w.rmse_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_dict(DuWork& w, const float* X, float* D, const int32_t* idx, const float* val,
+                             const int32_t* cnt, float* err, hipStream_t st) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(w.K, 256)), dim3(256), 0, st, w.usecount, (long)w.K, 0);
+    hipLaunchKernelGGL(usecount_kernel, dim3(blocks_for(w.N, 256)), dim3(256), 0, st, idx, val, cnt, w.N, w.T,
+                       w.usecount);
+    hipError_t e = launch_at_times_b(D, w.n, w.K, D, w.K, w.Gtmp, 0, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(clear_sweep_kernel, dim3(1), dim3(1024), 0, st, X, D, w.Gtmp, w.usecount, w.replaced, err, w.n,
+                       w.K, w.N, w.mu_thresh * w.mu_thresh, w.use_thresh);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_to_compact(const float* sel, const float* c, int K, int N, int T, int32_t* idx, float* val,
+                                   int32_t* cnt, hipStream_t st) {
+    hipLaunchKernelGGL(dense_to_compact_kernel, dim3(blocks_for(N, 4)), dim3(256), 0, st, sel, c, K, N, T, idx, val,
+                       cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_to_dense(const int32_t* idx, const float* val, const int32_t* cnt, int K, int N, int T,
+                                   float* out, hipStream_t st) {
+    hipLaunchKernelGGL(compact_to_dense_kernel, dim3(blocks_for(N, 4)), dim3(256), 0, st, idx, val, cnt, K, N, T,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float* p, long n, float v, hipStream_t st) {
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace ksvd

@@ -1,0 +1,212 @@
+// gemm.hip -- C = A^T B on CDNA4 MFMA (Gram G = D^T D and correlation
+// DtX = D^T X).  Replaces the cuBLAS ?gemm(T,N) of
+// Algorithms::Multiplication::AT_times_B (S/Multiplication.cu:40-63).
+//
+// Layout: A is rows x colsA, B is rows x colsB, both column-major, so both
+// operands are contiguous along the reduction dimension.  C is colsA x colsB
+// column-major: C[j*colsA + m] = <A(:,m), B(:,j)>.
+//
+// Block tile 128 (columns of B) x 128 (columns of A), BK = 32, 256 threads =
+// 4 waves in a 2 x 2 arrangement, each wave 64 x 64 = 2 x 2 MFMA 32x32 tiles.
+// The MFMA row operand is B (signals) and the column operand is A (atoms), so
+// each accumulator register holds 32 consecutive atoms of one signal: the
+// column-major store of C is one 128-byte segment per half-wave.
+//
+// Forms (MI355X dense peaks):
+//   FP32   v_mfma_f32_32x32x2_f32   (157 TF)  FP32 products, FP32 accumulate
+//   BF16   v_mfma_f32_32x32x16_bf16 (2.5 PF)  RNE-rounded BF16 operands
+//   BF16X3 three BF16 MFMAs: hi*hi + hi*lo + lo*hi with x = hi + lo, hi =
+//          bf16(x), lo = bf16(x - hi)  (Henry, Tang & Heinecke, ARITH 2019)
+// Operands are staged through LDS as FP32 and converted to BF16 in registers,
+// so one kernel serves all three forms.
+#include "common.hpp"
+
+namespace ksvd {
+
+namespace {
+
+constexpr int BM = 128;    // tile extent along colsB (MFMA rows)
+constexpr int BN = 128;    // tile extent along colsA (MFMA cols)
+constexpr int BK = 32;     // reduction chunk
+constexpr int PITCH = 36;  // LDS row pitch in floats (pads 128 B rows: no bank conflicts)
+
+__device__ __forceinline__ void to_bf16_split(const float4& p, const float4& q, bf16x8& hi, bf16x8& lo) {
+    float v[8] = {p.x, p.y, p.z, p.w, q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        __bf16 h = (__bf16)v[e];
+        hi[e] = h;
+        lo[e] = (__bf16)(v[e] - (float)h);
+    }
+}
+
+__device__ __forceinline__ void to_bf16(const float4& p, const float4& q, bf16x8& hi) {
+    hi[0] = (__bf16)p.x; hi[1] = (__bf16)p.y; hi[2] = (__bf16)p.z; hi[3] = (__bf16)p.w;
+    hi[4] = (__bf16)q.x; hi[5] = (__bf16)q.y; hi[6] = (__bf16)q.z; hi[7] = (__bf16)q.w;
+}
+
+// Loads one 128 x 32 tile (rows of the tile are columns of the matrix).
+__device__ __forceinline__ void load_tile(float* __restrict__ s, const float* __restrict__ M, int rows,
+                                          long col0, long cols, int t0, bool vec) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int q = tid + 256 * r;
+        const int row = q >> 3;
+        const int c4 = (q & 7) * 4;
+        const long col = col0 + row;
+        const int t = t0 + c4;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (col < cols) {
+            const float* src = M + col * (long)rows + t;
+            if (vec && t + 3 < rows) {
+                v = *reinterpret_cast<const float4*>(src);
+            } else {
+                if (t + 0 < rows) v.x = src[0];
+                if (t + 1 < rows) v.y = src[1];
+                if (t + 2 < rows) v.z = src[2];
+                if (t + 3 < rows) v.w = src[3];
+            }
+        }
+        *reinterpret_cast<float4*>(s + row * PITCH + c4) = v;
+    }
+}
+
+template <int FORM>
+__global__ __launch_bounds__(256) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
+                                                         const float* __restrict__ B, long colsB,
+                                                         float* __restrict__ C, int tilesA) {
+    __shared__ float sB[BM * PITCH];
+    __shared__ float sA[BN * PITCH];
+    const long blk = blockIdx.x;
+    // Consecutive blocks share the B tile (signals) and walk the A tiles
+    // (atoms), so each signal tile is fetched from HBM once and re-read from L2.
+    const int ta = (int)(blk % tilesA);
+    const long tb = blk / tilesA;
+    const long j0 = tb * BM;
+    const int m0 = ta * BN;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int wi = w & 1, wj = w >> 1;
+    const bool vec = (rows & 3) == 0;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+
+    const int li = lane & 31;
+    const int lk = lane >> 5;
+    for (int t0 = 0; t0 < rows; t0 += BK) {
+        load_tile(sB, B, rows, j0, colsB, t0, vec);
+        load_tile(sA, A, rows, m0, colsA, t0, vec);
+        __syncthreads();
+        if constexpr (FORM == 0) {
+#pragma unroll
+            for (int ks = 0; ks < BK; ks += 2) {
+                float ra[2], rb[2];
+#pragma unroll
+                for (int a = 0; a < 2; ++a) ra[a] = sB[(wi * 64 + a * 32 + li) * PITCH + ks + lk];
+#pragma unroll
+                for (int b = 0; b < 2; ++b) rb[b] = sA[(wj * 64 + b * 32 + li) * PITCH + ks + lk];
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ra[a], rb[b], acc[a][b], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < BK; ks += 16) {
+                bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    const float* p = sB + (wi * 64 + a * 32 + li) * PITCH + ks + lk * 8;
+                    float4 p0 = *reinterpret_cast<const float4*>(p);
+                    float4 p1 = *reinterpret_cast<const float4*>(p + 4);
+                    if constexpr (FORM == 2) to_bf16_split(p0, p1, ah[a], al[a]);
+                    else to_bf16(p0, p1, ah[a]);
+                }
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const float* p = sA + (wj * 64 + b * 32 + li) * PITCH + ks + lk * 8;
+                    float4 p0 = *reinterpret_cast<const float4*>(p);
+                    float4 p1 = *reinterpret_cast<const float4*>(p + 4);
+                    if constexpr (FORM == 2) to_bf16_split(p0, p1, bh[b], bl[b]);
+                    else to_bf16(p0, p1, bh[b]);
+                }
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        if constexpr (FORM == 2) {
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                        }
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+                    }
+            }
+        }
+        __syncthreads();
+    }
+
+    // Accumulator register r of lane l holds MFMA row 8*(r/4) + 4*(l/32) + r%4
+    // (a signal) and column l%32 (an atom).
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int m = m0 + wj * 64 + b * 32 + li;
+            if (m >= colsA) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const long j = j0 + wi * 64 + a * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
+                if (j < colsB) C[j * (long)colsA + m] = acc[a][b][r];
+            }
+        }
+}
+
+// normcols: one wave per atom.
+__global__ __launch_bounds__(256) void normcols_kernel(const float* __restrict__ Din, float* __restrict__ Dout,
+                                                       int n, int K) {
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= K) return;
+    const int lane = threadIdx.x & 63;
+    const float* d = Din + (long)k * n;
+    float s = 0.f;
+    for (int t = lane; t < n; t += 64) s += d[t] * d[t];
+    s = wave_sum(s);
+    // invSQRT: 1/sqrt(sum)  (S/ReductionBase.cuh:158-161), then diagMul
+    const float coeff = 1.0f / sqrtf(s);
+    for (int t = lane; t < n; t += 64) Dout[(long)k * n + t] = d[t] * coeff;
+}
+
+}  // namespace
+
+hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B, long colsB, float* C, int form,
+                             hipStream_t st) {
+    if (rows <= 0 || colsA <= 0 || colsB <= 0) return hipSuccess;
+    const int tilesA = (colsA + BN - 1) / BN;
+    const long tilesB = (colsB + BM - 1) / BM;
+    const long blocks = tilesB * tilesA;
+    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+    dim3 grid((unsigned)blocks), block(256);
+    switch (form) {
+        case 0: hipLaunchKernelGGL(at_times_b_kernel<0>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        case 1: hipLaunchKernelGGL(at_times_b_kernel<1>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        case 2: hipLaunchKernelGGL(at_times_b_kernel<2>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_normcols(const float* Din, float* Dout, int n, int K, hipStream_t st) {
+    dim3 grid((K + 3) / 4), block(256);
+    hipLaunchKernelGGL(normcols_kernel, grid, block, 0, st, Din, Dout, n, K);
+    return hipGetLastError();
+}
+
+}  // namespace ksvd

@@ -1,0 +1,389 @@
+"""GPU parity: every entry point of libksvd_amd.so, called through the C-ABI,
+against the FP64 oracle on the same seeded inputs.
+
+Tolerances (written here, per north-star precision rules):
+  GEMM  |C - C*| <= tol * (|A|^T |B|)  with tol = 2e-5 (FP32), 1e-2 (BF16), 1e-4 (BF16x3)
+  OMP   ordered supports identical except documented ties (oracle top-2 |alpha|
+        gap at the first differing step < 1e-4 * max|alpha0|); coefficient
+        relative error <= 1e-4 (FP32 GEMMs) on identical supports
+  DU    atoms and coefficients within 1e-4 (absolute, unit-norm atoms)
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from ksvd_amd import Engine, synth
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def cu(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).to(DEV)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def workload(cfg, N, seed=None):
+    wl = synth.make(cfg, N=N, seed=seed)
+    X32 = wl.X.numpy()
+    D32 = wl.D_init.numpy()
+    D32 = (D32 / np.linalg.norm(D32.astype(np.float64), axis=1, keepdims=True)).astype(np.float32)
+    return wl, X32, D32
+
+
+# ---- GEMM / normcols -----------------------------------------------------------------
+@pytest.mark.parametrize("form,tol", [("fp32", 2e-5), ("bf16", 1e-2), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("rows,colsA,colsB", [(64, 256, 1000), (240, 512, 777), (37, 130, 301)])
+def test_at_times_b_forms(o64, form, tol, rows, colsA, colsB):
+    say("gemm", form, rows, colsA, colsB)
+    rng = np.random.default_rng(rows + colsA)
+    A = rng.standard_normal((colsA, rows)).astype(np.float32)
+    B = rng.standard_normal((colsB, rows)).astype(np.float32)
+    eng = Engine(rows, colsA, colsB, 4, gemm_form=form)
+    C = host(eng.AT_times_B(cu(A), cu(B)))
+    ref = o64.at_times_b(A.astype(np.float64), B.astype(np.float64))
+    bound = np.abs(B).astype(np.float64) @ np.abs(A).astype(np.float64).T
+    assert np.all(np.abs(C - ref) <= tol * bound + 1e-30), np.max(np.abs(C - ref) / bound)
+    eng.close()
+
+
+def test_normcols(o64):
+    say("normcols")
+    rng = np.random.default_rng(0)
+    D = (rng.standard_normal((300, 240)) * 5).astype(np.float32)
+    eng = Engine(240, 300, 10, 4)
+    out = host(eng.normcols(cu(D)))
+    np.testing.assert_allclose(out, o64.normcols(D.astype(np.float64)), rtol=2e-6, atol=1e-7)
+    eng.close()
+
+
+# ---- Batch-OMP -----------------------------------------------------------------------------
+def _omp_case(o64, cfg, N, T=None, form="fp32", seed=None):
+    wl, X, D = workload(cfg, N, seed)
+    n, K = D.shape[1], D.shape[0]
+    T = T or synth.CONFIGS[cfg]["T"]
+    eng = Engine(n, K, N, T, gemm_form=form)
+    Xg, Dg = cu(X), cu(D)
+    G = eng.gram(Dg)
+    A0 = eng.correlate(Dg, Xg)
+    idx, val, cnt = eng.batch_omp_compact(A0, G)
+    torch.cuda.synchronize()
+    X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    A064 = o64.correlate(D64, X64)
+    ref = o64.batch_omp(A064, o64.gram(D64), T, gaps=True)
+    eng.close()
+    return (host(idx), host(val), host(cnt)), ref, A064
+
+
+def _check_omp(got, ref, A064, tie_rel=1e-4, coef_tol=1e-4, max_tie_frac=0.02):
+    idx, val, cnt = got
+    tol = tie_rel * np.abs(A064).max(axis=1)
+    eq, tie, mis, bad = H.support_parity(idx, cnt, ref["idx"], ref["cnt"], ref["gap"], tol)
+    N = len(cnt)
+    say(f"supports: equal {eq}/{N}, documented ties {tie}, mismatches {mis} {bad[:5]}")
+    assert mis == 0
+    assert tie <= max_tie_frac * N
+    same = (cnt == ref["cnt"]) & np.all(idx == ref["idx"], axis=1)
+    num = np.linalg.norm(val[same] - ref["val"][same], axis=1)
+    den = np.linalg.norm(ref["val"][same], axis=1) + 1e-30
+    rel = num / den
+    say(f"coef rel err: max {rel.max():.3e} median {np.median(rel):.3e}")
+    assert rel.max() <= coef_tol
+    return eq / N, rel
+
+
+@pytest.mark.parametrize("cfg,N", [("C0", 4000), ("C1", 2000), ("C2", 1024)])
+def test_batch_omp_parity_fp32(o64, cfg, N):
+    say("batch_omp", cfg, N)
+    got, ref, A064 = _omp_case(o64, cfg, N)
+    _check_omp(got, ref, A064)
+
+
+def test_batch_omp_bf16x3_parity_c2(o64):
+    say("batch_omp bf16x3 C2 subset")
+    got, ref, A064 = _omp_case(o64, "C2", 2048, form="bf16x3")
+    frac, rel = _check_omp(got, ref, A064, tie_rel=1e-3, coef_tol=1e-3, max_tie_frac=0.05)
+    say(f"bf16x3 identical-support fraction {frac:.4f}")
+
+
+def test_batch_omp_dense_reference_layout(o64):
+    """SELECTED_ATOMS slot map and c exactly as parOMP writes them, including the
+    pivot-break slot that points at a stale c entry."""
+    say("batch_omp dense")
+    E = np.eye(8)
+    D = np.stack([E[0], E[0], E[1], E[2]])
+    G = (D @ D.T).astype(np.float32)
+    A0 = np.array([[3.5, 4.0, 0.1, 0.1], [0.0, 0.0, 2.0, 1.0]], np.float32)
+    eng = Engine(8, 4, 2, 3)
+    sel, c = eng.batch_omp(cu(A0), cu(G), c=cu(np.full((2, 3), 7.0, np.float32)))
+    ref = o64.batch_omp(A0.astype(np.float64), G.astype(np.float64), 3, dense=True, c_init=np.full((2, 3), 7.0))
+    np.testing.assert_array_equal(host(sel), ref["sel"])
+    np.testing.assert_allclose(host(c), ref["c"], rtol=1e-6)
+    eng.close()
+    # larger random case: slot map identical where supports agree
+    wl, X, Dm = workload("C0", 1000)
+    eng = Engine(64, 256, 1000, 8)
+    G = eng.gram(cu(Dm))
+    A0g = eng.correlate(cu(Dm), cu(X))
+    sel, c = eng.batch_omp(A0g, G)
+    idx, val, cnt = eng.batch_omp_compact(A0g, G)
+    sel, c, idx, cnt, val = host(sel), host(c), host(idx), host(cnt), host(val)
+    for s in range(1000):
+        k = np.nonzero(sel[s])[0]
+        assert len(k) == cnt[s]
+        assert np.array_equal(sel[s, idx[s, :cnt[s]]], np.arange(1, cnt[s] + 1))
+        np.testing.assert_array_equal(c[s, :cnt[s]], val[s, :cnt[s]])
+    eng.close()
+
+
+def test_batch_omp_eps_parity(o64):
+    say("batch_omp_eps C3 subset")
+    wl, X, D = workload("C3", 3000)
+    n, K, N, Tmax = 128, 512, 3000, 48
+    eng = Engine(n, K, N, Tmax)
+    Xg, Dg = cu(X), cu(D)
+    G = eng.gram(Dg)
+    A0 = eng.correlate(Dg, Xg)
+    xn2 = (Xg.double() ** 2).sum(1).float()
+    eps2 = cu(wl.eps2.numpy())
+    idx, val, cnt = eng.batch_omp_eps(A0, G, xn2, eps2)
+    got = (host(idx), host(val), host(cnt))
+    X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    A064 = o64.correlate(D64, X64)
+    ref = o64.batch_omp(A064, o64.gram(D64), Tmax, xnorm2=(X64 ** 2).sum(1), eps2=wl.eps2.numpy().astype(np.float64),
+                        gaps=True)
+    # the eps stop compares ||r||^2 with eps^2 in FP64 on FP32 inputs: a signal
+    # may stop one atom apart when ||r||^2 is within FP32 error of eps^2
+    idx_g, val_g, cnt_g = got
+    tol = 1e-4 * np.abs(A064).max(axis=1)
+    eq, tie, mis, bad = H.support_parity(idx_g, cnt_g, ref["idx"], ref["cnt"], ref["gap"], tol)
+    near = 0
+    for s in bad:
+        c = min(cnt_g[s], ref["cnt"][s])
+        if abs(int(cnt_g[s]) - int(ref["cnt"][s])) == 1 and np.array_equal(idx_g[s, :c], ref["idx"][s, :c]):
+            near += 1
+    say(f"eps supports: equal {eq}, ties {tie}, off-by-one stops {near}, other {mis - near}")
+    assert mis - near == 0 and near <= 0.01 * N
+    assert np.all(cnt_g <= Tmax) and cnt_g.min() >= 1
+    eng.close()
+
+
+# ---- dictionary update / error / clear --------------------------------------------------------
+def _du_inputs(o64, cfg, N, drop_atom=True, seed=None):
+    wl, X, D = workload(cfg, N, seed)
+    X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    T = synth.CONFIGS[cfg]["T"]
+    r = o64.batch_omp(o64.correlate(D64, X64), o64.gram(D64), T)
+    idx, val, cnt = r["idx"].copy(), r["val"].astype(np.float32), r["cnt"].copy()
+    K = D.shape[0]
+    if drop_atom:  # make one atom unused to exercise the special case
+        used = np.bincount(idx[idx >= 0], minlength=K)
+        p = int(np.argmin(used))
+        for s in range(N):
+            keep = [j for j in range(cnt[s]) if idx[s, j] != p]
+            i2, v2 = idx[s, keep].copy(), val[s, keep].copy()
+            idx[s] = -1
+            val[s] = 0
+            idx[s, :len(keep)] = i2
+            val[s, :len(keep)] = v2
+            cnt[s] = len(keep)
+    return X, D, idx, val, cnt, T
+
+
+@pytest.mark.parametrize("cfg,N,max_signals", [("C0", 3000, 5000), ("C1", 1500, 700)])
+def test_dictionary_update_parity(o64, cfg, N, max_signals):
+    say("dictionary_update", cfg, N)
+    X, D, idx, val, cnt, T = _du_inputs(o64, cfg, N)
+    n, K = D.shape[1], D.shape[0]
+    perm = np.random.default_rng(1).permutation(K).astype(np.uint32)
+    eng = Engine(n, K, N, T, max_signals=max_signals)
+    Dg, valg = cu(D), cu(val)
+    eng.du_init()
+    eng.updateDictionary(cu(X), Dg, cu(idx, torch.int32), valg, cu(cnt, torch.int32), perm)
+    D1, v1 = host(Dg), host(valg)
+    Dr, vr, st = o64.dict_update(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt, perm,
+                                 max_signals=max_signals)
+    assert st["usc"][0] >= 1  # the special case ran
+    say("D max abs diff", np.abs(D1 - Dr).max(), "val max abs diff", np.abs(v1 - vr).max())
+    np.testing.assert_allclose(D1, Dr, atol=1e-4)
+    np.testing.assert_allclose(v1, vr, atol=1e-4 * max(1.0, np.abs(vr).max()))
+    eng.close()
+
+
+def test_dictionary_update_dense_layout(o64):
+    say("dictionary_update dense")
+    wl, X, D = workload("C0", 1200)
+    eng = Engine(64, 256, 1200, 8)
+    G = eng.gram(cu(D))
+    A0 = eng.correlate(cu(D), cu(X))
+    sel, c = eng.batch_omp(A0, G)
+    idx, val, cnt = eng.batch_omp_compact(A0, G)
+    perm = np.random.default_rng(2).permutation(256).astype(np.uint32)
+    D1, D2 = cu(D), cu(D)
+    eng.du_init()
+    eng.updateDictionary_dense(cu(X), D1, sel, c, perm)
+    eng.du_init()
+    eng.updateDictionary(cu(X), D2, idx, val, cnt, perm)
+    np.testing.assert_allclose(host(D1), host(D2), atol=1e-5)
+    dense = H.dense_gamma(host(idx), host(val), host(cnt), 256)
+    np.testing.assert_allclose(host(sel), dense, atol=1e-5)
+    eng.close()
+
+
+def test_error_and_clear_dict_parity(o64):
+    say("error + clear_dict")
+    X, D, idx, val, cnt, T = _du_inputs(o64, "C0", 2000, drop_atom=False)
+    n, K, N = 64, 256, 2000
+    D = D.copy()
+    D[1] = D[0] * 0.98 + 0.02 * D[2]
+    D[1] /= np.linalg.norm(D[1])
+    eng = Engine(n, K, N, T)
+    eng.du_init()
+    Xg, Dg = cu(X), cu(D)
+    idg, vg, cg = cu(idx, torch.int32), cu(val), cu(cnt, torch.int32)
+    rmse, err = eng.errorComputation(Xg, Dg, idg, vg, cg)
+    r_ref, e_ref = o64.error(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt)
+    assert abs(rmse - r_ref) <= 1e-6 * r_ref
+    np.testing.assert_allclose(host(err), e_ref, rtol=1e-4, atol=1e-6)
+    eng.clearDict(Xg, Dg, idg, vg, cg, err)
+    Dr, er, flags = o64.clear_dict(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt,
+                                   np.zeros(K, np.int32), host(err).astype(np.float64))
+    assert flags[0] == 1
+    np.testing.assert_allclose(host(Dg), Dr, atol=1e-5)
+    eng.close()
+
+
+def test_execute_iterations_parity(o64):
+    say("execute_iterations C0")
+    wl = synth.make("C0", N=3000)
+    X, D0 = wl.X.numpy(), wl.D_init.numpy()
+    eng = Engine(64, 256, 3000, 8, perm_seed=11)
+    D, idx, val, cnt, rmse = eng.ExecuteIterations(cu(X), cu(D0), 4)
+    ref = o64.ksvd(X.astype(np.float64), D0.astype(np.float64), 8, 4, seed=11)
+    say("rmse gpu", rmse, "oracle", ref["rmse"])
+    np.testing.assert_allclose(rmse[0], ref["rmse"][0], rtol=1e-4)
+    np.testing.assert_allclose(rmse, ref["rmse"], rtol=2e-3)
+    assert np.all(np.diff(rmse) < 0)
+    eng.close()
+    # one iteration: dictionaries agree closely
+    eng = Engine(64, 256, 3000, 8, perm_seed=11)
+    D, idx, val, cnt, rmse = eng.ExecuteIterations(cu(X), cu(D0), 1)
+    ref = o64.ksvd(X.astype(np.float64), D0.astype(np.float64), 8, 1, seed=11)
+    diff = np.abs(host(D) - ref["D"]).max(axis=1)
+    say("1-iteration D column max diff: median", np.median(diff), "max", diff.max())
+    assert np.median(diff) < 1e-4
+    eng.close()
+
+
+def test_reference_fixture_structure_gpu():
+    """Reference D.mat (300 x 9), T=6, 30 iterations: every column of Gamma has
+    exactly 6 nonzeros and every atom is used, as in the reference's CUDA_Gamma."""
+    say("reference fixture structure")
+    Dref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_D.npy")).T.astype(np.float32).copy()
+    rng = np.random.default_rng(10)
+    N = 3600
+    Dn = Dref / np.linalg.norm(Dref, axis=1, keepdims=True)
+    S = np.stack([rng.choice(9, 6, replace=False) for _ in range(N)])
+    Xc = np.einsum("st,stn->sn", rng.standard_normal((N, 6)), Dn[S])
+    X = (Xc + rng.standard_normal(Xc.shape) * np.linalg.norm(Xc, axis=1, keepdims=True) / np.sqrt(300) * 0.1)
+    eng = Engine(300, 9, N, 6)
+    D, idx, val, cnt, rmse = eng.ExecuteIterations(cu(X.astype(np.float32)), cu(Dref), 30)
+    cnt, idx = host(cnt), host(idx)
+    assert np.all(cnt == 6)
+    assert np.all(np.bincount(idx.ravel(), minlength=9) > 0)
+    assert np.all(np.isfinite(host(val)))
+    eng.close()
+
+
+# ---- adjoint ---------------------------------------------------------------------------------
+def test_omp_adjoint_parity_and_finite_differences(o64):
+    say("omp_adjoint C4 subset")
+    wl = synth.make("C4", N=512)
+    X = wl.X.numpy()
+    D = wl.D_init.numpy()
+    D = (D / np.linalg.norm(D.astype(np.float64), axis=1, keepdims=True)).astype(np.float32)
+    n, K, N, T = 64, 256, 512, 10
+    eng = Engine(n, K, N, T)
+    Xg, Dg = cu(X), cu(D)
+    idx, val, cnt = eng.batch_omp_compact(eng.correlate(Dg, Xg), eng.gram(Dg))
+    W = wl.W.numpy()
+    y = wl.y.numpy()
+    idx_h, cnt_h, val_h = host(idx), host(cnt), host(val)
+    # loss ||W gamma - y||^2 with gamma the dense K-vector: g = 2 W^T (W gamma - y) on the support
+    Gd = H.dense_gamma(idx_h, val_h, cnt_h, K)
+    R = Gd @ W.T - y
+    gdense = 2 * R @ W
+    gval = np.zeros((N, T), np.float32)
+    for s in range(N):
+        gval[s, :cnt_h[s]] = gdense[s, idx_h[s, :cnt_h[s]]]
+    gD, gX = eng.omp_adjoint(Dg, Xg, idx, val, cnt, cu(gval))
+    gD, gX = host(gD), host(gX)
+    X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    val64 = o64.ls_fixed(D64, X64, idx_h, cnt_h)
+    gD_ref, gX_ref = o64.ls_adjoint(D64, X64, idx_h, cnt_h, val64, gval.astype(np.float64))
+    scaleD, scaleX = np.abs(gD_ref).max(), np.abs(gX_ref).max()
+    say("adjoint max rel err D", np.abs(gD - gD_ref).max() / scaleD, "X", np.abs(gX - gX_ref).max() / scaleX)
+    assert np.abs(gD - gD_ref).max() <= 1e-4 * scaleD
+    assert np.abs(gX - gX_ref).max() <= 1e-4 * scaleX
+
+    def loss(Dm, Xm):
+        v = o64.ls_fixed(Dm, Xm, idx_h, cnt_h)
+        Gm = H.dense_gamma(idx_h, v, cnt_h, K)
+        return float(np.sum((Gm @ W.T - y) ** 2))
+
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    errs = []
+    for _ in range(64):
+        s = int(rng.integers(N))
+        j = int(rng.integers(cnt_h[s]))
+        k, t = int(idx_h[s, j]), int(rng.integers(n))
+        Dp, Dm_ = D64.copy(), D64.copy()
+        Dp[k, t] += h
+        Dm_[k, t] -= h
+        fd = (loss(Dp, X64) - loss(Dm_, X64)) / (2 * h)
+        errs.append(abs(fd - gD[k, t]) / max(abs(fd), 1e-3 * scaleD))
+    say("FD rel err (64 entries of D): max", max(errs))
+    assert max(errs) <= 1e-3
+    eng.close()
+
+
+# ---- full-size properties -------------------------------------------------------------------
+def test_c2_full_size_properties(o64):
+    """N = 10^6 signals of C2: structural invariants over every signal, and
+    oracle parity on a seeded sample of 256 signals."""
+    say("C2 full size")
+    wl = synth.make("C2", device=DEV)
+    X, D0 = wl.X, wl.D_init
+    n, K, N, T = 256, 1024, X.shape[0], 32
+    eng = Engine(n, K, N, T)
+    D = eng.normcols(D0)
+    G = eng.gram(D)
+    A0 = eng.correlate(D, X)
+    idx, val, cnt = eng.batch_omp_compact(A0, G)
+    torch.cuda.synchronize()
+    c = cnt.cpu().numpy()
+    # the K signals that became atoms stop after one atom; the rest take T
+    assert np.all((c == T) | (c == 1)) and np.sum(c == 1) <= K
+    ii = idx.cpu().numpy()
+    assert np.all((ii[:, 0] >= 0) & (ii[:, 0] < K))
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(N, 256, replace=False))
+    Xs = X[sample].double().cpu().numpy()
+    Dh = D.double().cpu().numpy()
+    ref = o64.batch_omp(o64.correlate(Dh, Xs), o64.gram(Dh), T, gaps=True)
+    got = (ii[sample], val.cpu().numpy()[sample], c[sample])
+    _check_omp(got, ref, o64.correlate(Dh, Xs))
+    eng.close()
