@@ -52,10 +52,14 @@ enum {
     KSVD_FORM_BF16X3 = 2  /* split-BF16 x3: hi*hi + hi*lo + lo*hi, FP32 acc */
 };
 
-/* Batch-OMP kernel variant. */
+/* Batch-OMP kernel variant (entry points that receive D and X). */
 enum {
-    KSVD_OMP_AUTO = 0,  /* choose from the autotuner cache / heuristics */
-    KSVD_OMP_WAVE = 1   /* one wave64 per signal, alpha in VGPRs, L in LDS */
+    KSVD_OMP_AUTO = 0,  /* MFMA form when n <= 256, else the wave form          */
+    KSVD_OMP_WAVE = 1,  /* one wave64 per signal: alpha = alpha0 - G(:,I) gamma  */
+                        /* in VGPRs over coalesced G columns, L in LDS           */
+    KSVD_OMP_MFMA = 2   /* 64-signal tiles: alpha = D^T r on fp16 MFMA with a    */
+                        /* rigorous error bound, exact FP32 re-evaluation of the */
+                        /* candidates within it; n <= 256                        */
 };
 
 /*
@@ -128,12 +132,27 @@ int ksvd_batch_omp_compact(ksvd_handle* h, const float* DtX, const float* G, int
 int ksvd_batch_omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G,
                                 int32_t* idx, float* val, int32_t* cnt);
 
+/* OMP::BatchOMP with the reference output layout: SELECTED_ATOMS slot map and
+ * c (slots beyond a signal's count untouched).  Uses the configured variant. */
+int ksvd_batch_omp_from_signals_dense(ksvd_handle* h, const float* D, const float* X,
+                                      const float* G, float* selected_atoms, float* c);
+
 /* Error-constrained Batch-OMP (no reference counterpart beyond the comment at
  * S/KSVD.cu:79): per signal, stop when ||r||^2 = ||x||^2 - alpha0_I^T gamma_I
  * (evaluated in FP64) <= eps2[s], or at T atoms (T_max).  xnorm2[s] = ||x_s||^2.
  * Runs as one persistent launch with work stealing over signals. */
 int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const float* xnorm2,
                        const float* eps2, int32_t* idx, float* val, int32_t* cnt);
+
+/* Error-constrained Batch-OMP from D and X with the configured variant:
+ * stop when ||r||^2 <= eps2[s] (FP64 sum) or at T atoms. */
+int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float* X,
+                                    const float* G, const float* eps2, int32_t* idx, float* val,
+                                    int32_t* cnt);
+
+/* Number of signal-steps of the last MFMA-form call whose candidate lists were
+ * incomplete and fell back to FP32 correlations with every atom (synchronises). */
+int ksvd_omp_fallbacks(ksvd_handle* h, int32_t* count);
 
 /* ---- dictionary update / error / clear ------------------------------------ */
 

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(HERE, "libksvd_amd.so")
 
 FORM_FP32, FORM_BF16, FORM_BF16X3 = 0, 1, 2
 FORMS = {"fp32": FORM_FP32, "bf16": FORM_BF16, "bf16x3": FORM_BF16X3}
+OMP_AUTO, OMP_WAVE, OMP_MFMA = 0, 1, 2
+OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "mfma": OMP_MFMA}
 
 KSVD_OK = 0
 STATUS = {0: "KSVD_OK", -1: "KSVD_ERR_INVALID", -2: "KSVD_ERR_LIMIT", -3: "KSVD_ERR_HIP", -4: "KSVD_ERR_ALLOC",
@@ -21,7 +23,8 @@ STATUS = {0: "KSVD_OK", -1: "KSVD_ERR_INVALID", -2: "KSVD_ERR_LIMIT", -3: "KSVD_
 EXPORTS = [
     "ksvd_config_default", "ksvd_create", "ksvd_destroy", "ksvd_set_stream", "ksvd_last_error", "ksvd_version",
     "ksvd_sync", "ksvd_normcols", "ksvd_at_times_b", "ksvd_gram", "ksvd_correlate", "ksvd_batch_omp",
-    "ksvd_batch_omp_compact", "ksvd_batch_omp_from_signals", "ksvd_batch_omp_eps", "ksvd_du_init",
+    "ksvd_batch_omp_compact", "ksvd_batch_omp_from_signals", "ksvd_batch_omp_from_signals_dense",
+    "ksvd_batch_omp_eps", "ksvd_batch_omp_eps_from_signals", "ksvd_omp_fallbacks", "ksvd_du_init",
     "ksvd_dictionary_update", "ksvd_dictionary_update_dense", "ksvd_error", "ksvd_clear_dict",
     "ksvd_execute_iterations", "ksvd_perm_sequence", "ksvd_omp_adjoint",
 ]
@@ -73,7 +76,10 @@ def load() -> C.CDLL:
     sig("ksvd_batch_omp", I, H, P, P, P, P)
     sig("ksvd_batch_omp_compact", I, H, P, P, P, P, P)
     sig("ksvd_batch_omp_from_signals", I, H, P, P, P, P, P, P)
+    sig("ksvd_batch_omp_from_signals_dense", I, H, P, P, P, P, P)
     sig("ksvd_batch_omp_eps", I, H, P, P, P, P, P, P, P)
+    sig("ksvd_batch_omp_eps_from_signals", I, H, P, P, P, P, P, P, P)
+    sig("ksvd_omp_fallbacks", I, H, C.POINTER(C.c_int32))
     sig("ksvd_du_init", I, H)
     sig("ksvd_dictionary_update", I, H, P, P, P, P, P, P)
     sig("ksvd_dictionary_update_dense", I, H, P, P, P, P, P)
@@ -102,5 +108,7 @@ def default_config(**kw) -> Config:
     for k, v in kw.items():
         if k == "gemm_form" and isinstance(v, str):
             v = FORMS[v]
+        if k == "omp_variant" and isinstance(v, str):
+            v = OMP_VARIANTS[v]
         setattr(cfg, k, v)
     return cfg

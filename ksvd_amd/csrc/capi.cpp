@@ -36,7 +36,25 @@ struct ksvd_handle {
     int rmse_cap = 0;
     uint32_t* perm_all = nullptr;
     int perm_cap = 0;
+    void* fast = nullptr;      // MFMA-form Batch-OMP scratch
+    int fast_grid = 0;
+    float* xnorm2 = nullptr;   // N (wave-form error mode from signals)
 };
+
+// ||x_s||^2 per signal (one wave per signal).
+__global__ __launch_bounds__(256) void sumsq_cols_kernel(const float* __restrict__ X, int n, int N,
+                                                         float* __restrict__ out) {
+    const long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= N) return;
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    for (int t = lane; t < n; t += 64) {
+        const double v = X[s * n + t];
+        acc += v * v;
+    }
+    acc = ksvd::wave_sum(acc);
+    if (lane == 0) out[s] = (float)acc;
+}
 
 namespace {
 
@@ -85,6 +103,7 @@ int check_cfg(ksvd_handle* h, const ksvd_config* c) {
     if (c->n > 4096) return fail(h, KSVD_ERR_LIMIT, "n=%d exceeds 4096", c->n);
     if ((long)c->N * c->T > 0x7fffffffL) return fail(h, KSVD_ERR_LIMIT, "N*T exceeds 2^31");
     if (c->gemm_form < 0 || c->gemm_form > 2) return fail(h, KSVD_ERR_INVALID, "gemm_form=%d", c->gemm_form);
+    if (c->omp_variant < 0 || c->omp_variant > 2) return fail(h, KSVD_ERR_INVALID, "omp_variant=%d", c->omp_variant);
     if (c->mu_thresh < 0) return fail(h, KSVD_ERR_INVALID, "mu_thresh must be >= 0");
     if (c->max_signals < 1) return fail(h, KSVD_ERR_INVALID, "max_signals must be >= 1");
     return KSVD_OK;
@@ -159,6 +178,47 @@ ksvd::OmpArgs omp_args(ksvd_handle* h, const float* DtX, const float* G) {
     a.T = h->cfg.T;
     a.pivot_fp64 = (float)h->cfg.pivot_fp64;
     return a;
+}
+
+bool use_fast(const ksvd_handle* h) {
+    const int v = h->cfg.omp_variant;
+    if (v == KSVD_OMP_WAVE) return false;
+    return h->cfg.n <= 256;
+}
+
+// OMP::BatchOMP from D and X with the configured variant (compact output).
+int omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G, const float* eps2,
+                     int32_t* idx, float* val, int32_t* cnt) {
+    const ksvd_config& c = h->cfg;
+    if (use_fast(h)) {
+        if (!h->fast) {
+            h->fast_grid = ksvd::fast_grid();
+            void* p = nullptr;
+            TRY(alloc(h, reinterpret_cast<char**>(&p), ksvd::fast_scratch_bytes(c.n, c.K, c.T, h->fast_grid),
+                      "fast scratch"));
+            h->fast = p;
+        }
+        return hip_check(h,
+                         ksvd::launch_batch_omp_fast(X, D, G, c.n, c.K, c.N, c.T, eps2, idx, val, cnt,
+                                                     (float)c.pivot_fp64, h->fast, h->fast_grid, h->counter,
+                                                     h->stream),
+                         "batch_omp (mfma)");
+    }
+    TRY(alloc(h, &h->dtx, (size_t)c.K * c.N, "DtX"));
+    HIPTRY(h, ksvd::launch_at_times_b(D, c.n, c.K, X, c.N, h->dtx, c.gemm_form, h->stream), "correlate");
+    ksvd::OmpArgs a = omp_args(h, h->dtx, G);
+    a.idx = idx;
+    a.val = val;
+    a.cnt = cnt;
+    if (eps2) {
+        TRY(alloc(h, &h->xnorm2, c.N, "xnorm2"));
+        hipLaunchKernelGGL(sumsq_cols_kernel, dim3((unsigned)((c.N + 3) / 4)), dim3(256), 0, h->stream, X, c.n, c.N,
+                           h->xnorm2);
+        a.xnorm2 = h->xnorm2;
+        a.eps2 = eps2;
+        a.work_counter = h->counter;
+    }
+    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp (wave)");
 }
 
 }  // namespace
@@ -272,9 +332,41 @@ int ksvd_batch_omp_from_signals(ksvd_handle* h, const float* D, const float* X, 
                                 float* val, int32_t* cnt) {
     if (!h) return KSVD_ERR_NULL;
     if (!D || !X || !G || !idx || !val || !cnt) return fail(h, KSVD_ERR_NULL, "batch_omp_from_signals: null pointer");
-    TRY(alloc(h, &h->dtx, (size_t)h->cfg.K * h->cfg.N, "DtX"));
-    TRY(ksvd_correlate(h, D, X, h->dtx));
-    return ksvd_batch_omp_compact(h, h->dtx, G, idx, val, cnt);
+    if (h->cfg.omp_variant == KSVD_OMP_MFMA && h->cfg.n > 256)
+        return fail(h, KSVD_ERR_LIMIT, "MFMA Batch-OMP needs n <= 256 (n=%d)", h->cfg.n);
+    return omp_from_signals(h, D, X, G, nullptr, idx, val, cnt);
+}
+
+int ksvd_batch_omp_from_signals_dense(ksvd_handle* h, const float* D, const float* X, const float* G,
+                                      float* selected_atoms, float* c) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D || !X || !G || !selected_atoms || !c)
+        return fail(h, KSVD_ERR_NULL, "batch_omp_from_signals_dense: null pointer");
+    const ksvd_config& cf = h->cfg;
+    TRY(alloc(h, &h->cidx, (size_t)cf.N * cf.T, "cidx"));
+    TRY(alloc(h, &h->cval, (size_t)cf.N * cf.T, "cval"));
+    TRY(alloc(h, &h->ccnt, cf.N, "ccnt"));
+    TRY(ksvd_batch_omp_from_signals(h, D, X, G, h->cidx, h->cval, h->ccnt));
+    return hip_check(h, ksvd::launch_compact_to_slotmap(h->cidx, h->cval, h->ccnt, cf.K, cf.N, cf.T, selected_atoms, c,
+                                                        h->stream),
+                     "compact_to_slotmap");
+}
+
+int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G,
+                                    const float* eps2, int32_t* idx, float* val, int32_t* cnt) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D || !X || !G || !eps2 || !idx || !val || !cnt)
+        return fail(h, KSVD_ERR_NULL, "batch_omp_eps_from_signals: null pointer");
+    if (h->cfg.omp_variant == KSVD_OMP_MFMA && h->cfg.n > 256)
+        return fail(h, KSVD_ERR_LIMIT, "MFMA Batch-OMP needs n <= 256 (n=%d)", h->cfg.n);
+    return omp_from_signals(h, D, X, G, eps2, idx, val, cnt);
+}
+
+int ksvd_omp_fallbacks(ksvd_handle* h, int32_t* count) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!count) return fail(h, KSVD_ERR_NULL, "omp_fallbacks: null pointer");
+    HIPTRY(h, hipMemcpyAsync(count, h->counter + 1, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream), "download");
+    return hip_check(h, hipStreamSynchronize(h->stream), "sync");
 }
 
 int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const float* xnorm2, const float* eps2,
@@ -363,7 +455,6 @@ int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, f
     if (iterations < 0) return fail(h, KSVD_ERR_INVALID, "iterations=%d", iterations);
     const ksvd_config& c = h->cfg;
     TRY(ensure_du(h));
-    TRY(alloc(h, &h->dtx, (size_t)c.K * c.N, "DtX"));
     TRY(alloc(h, &h->gram, (size_t)c.K * c.K, "G"));
     TRY(alloc(h, &h->errv, c.N, "err"));
     if (iterations > h->rmse_cap || iterations > h->perm_cap) {
@@ -386,12 +477,7 @@ int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, f
     double* rmse_slot = h->du.rmse_dev;
     for (int it = 0; it < iterations; ++it) {
         HIPTRY(h, ksvd::launch_at_times_b(D_out, c.n, c.K, D_out, c.K, h->gram, c.gemm_form, h->stream), "gram");
-        HIPTRY(h, ksvd::launch_at_times_b(D_out, c.n, c.K, X, c.N, h->dtx, c.gemm_form, h->stream), "correlate");
-        ksvd::OmpArgs a = omp_args(h, h->dtx, h->gram);
-        a.idx = idx;
-        a.val = val;
-        a.cnt = cnt;
-        HIPTRY(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp");
+        TRY(omp_from_signals(h, D_out, X, h->gram, nullptr, idx, val, cnt));
         HIPTRY(h, ksvd::launch_du_init(h->du, h->stream), "du_init");
         h->du.perm_dev = h->perm_all + (size_t)it * c.K;
         HIPTRY(h, ksvd::launch_du_sweep(h->du, X, D_out, idx, val, cnt, h->stream), "dictionary_update");

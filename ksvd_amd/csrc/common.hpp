@@ -112,6 +112,15 @@ struct OmpArgs {
 };
 hipError_t launch_batch_omp(const OmpArgs& a, hipStream_t st);
 
+// Batch-OMP with the correlation update on MFMA (omp_fast.hip); n <= 256.
+size_t fast_scratch_bytes(int n, int K, int T, int grid);
+int fast_grid();
+hipError_t launch_batch_omp_fast(const float* X, const float* D, const float* G, int n, int K, int N, int T,
+                                 const float* eps2, int32_t* idx, float* val, int32_t* cnt, float pivot_fp64,
+                                 void* scratch, int grid, int32_t* counters, hipStream_t st);
+hipError_t launch_compact_to_slotmap(const int32_t* idx, const float* val, const int32_t* cnt, int K, int N, int T,
+                                     float* sel, float* c, hipStream_t st);
+
 // Dictionary update / error / clear (dict_update.hip)
 struct DuWork {
     int n, K, N, T, max_signals, use_thresh;

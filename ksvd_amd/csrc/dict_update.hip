@@ -593,6 +593,27 @@ __global__ __launch_bounds__(256) void compact_to_dense_kernel(const int32_t* __
     }
 }
 
+// compact -> reference slot map (K x N, slot+1 or 0) and c (T x N); slots of c
+// beyond the count are left untouched, as parOMP leaves them.
+__global__ __launch_bounds__(256) void compact_to_slotmap_kernel(const int32_t* __restrict__ idx,
+                                                                 const float* __restrict__ val,
+                                                                 const int32_t* __restrict__ cnt, int K, int N, int T,
+                                                                 float* __restrict__ sel, float* __restrict__ c) {
+    const long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= N) return;
+    const int lane = threadIdx.x & 63;
+    const int cc = cnt[s];
+    const int il = (lane < cc) ? idx[s * T + lane] : -1;
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        const int k = k0 + lane;
+        float v = 0.f;
+        for (int j = 0; j < cc; ++j)
+            if (readlane(il, j) == k) v = (float)(j + 1);
+        if (k < K) sel[s * K + k] = v;
+    }
+    if (lane < cc) c[s * T + lane] = val[s * T + lane];
+}
+
 }  // namespace
 
 // ---- launchers ---------------------------------------------------------------
@@ -670,6 +691,13 @@ hipError_t launch_compact_to_dense(const int32_t* idx, const float* val, const i
                                    float* out, hipStream_t st) {
     hipLaunchKernelGGL(compact_to_dense_kernel, dim3(blocks_for(N, 4)), dim3(256), 0, st, idx, val, cnt, K, N, T,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_to_slotmap(const int32_t* idx, const float* val, const int32_t* cnt, int K, int N, int T,
+                                     float* sel, float* c, hipStream_t st) {
+    hipLaunchKernelGGL(compact_to_slotmap_kernel, dim3(blocks_for(N, 4)), dim3(256), 0, st, idx, val, cnt, K, N, T,
+                       sel, c);
     return hipGetLastError();
 }
 

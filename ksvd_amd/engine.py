@@ -43,12 +43,12 @@ def _check_shape(t, shape, dtype, name):
 class Engine:
     """One engine handle (ksvd_create) bound to the current torch stream."""
 
-    def __init__(self, n, K, N, T, *, iterations=30, gemm_form="fp32", mu_thresh=0.8, use_thresh=4,
-                 max_signals=5000, pivot_fp64=1e-3, perm_seed=0):
+    def __init__(self, n, K, N, T, *, iterations=30, gemm_form="fp32", omp_variant="auto", mu_thresh=0.8,
+                 use_thresh=4, max_signals=5000, pivot_fp64=1e-3, perm_seed=0):
         self.lib = _lib.load()
         self.cfg = _lib.default_config(n=n, K=K, N=N, T=T, iterations=iterations, gemm_form=gemm_form,
-                                       mu_thresh=mu_thresh, use_thresh=use_thresh, max_signals=max_signals,
-                                       pivot_fp64=pivot_fp64, perm_seed=perm_seed)
+                                       omp_variant=omp_variant, mu_thresh=mu_thresh, use_thresh=use_thresh,
+                                       max_signals=max_signals, pivot_fp64=pivot_fp64, perm_seed=perm_seed)
         self.n, self.K, self.N, self.T = n, K, N, T
         h = C.c_void_p()
         rc = self.lib.ksvd_create(C.byref(h), C.byref(self.cfg))
@@ -149,6 +149,27 @@ class Engine:
         idx, val, cnt = self.empty_compact() if out is None else out
         self._rc(self.lib.ksvd_batch_omp_from_signals(self.h, _p(D), _p(X), _p(G), _p(idx), _p(val), _p(cnt)))
         return idx, val, cnt
+
+    def BatchOMP_dense(self, D, X, G, selected_atoms=None, c=None):
+        """OMP::BatchOMP with the reference layout (SELECTED_ATOMS slot map, c)."""
+        if selected_atoms is None:
+            selected_atoms = torch.empty((self.N, self.K), dtype=torch.float32, device=X.device)
+        if c is None:
+            c = torch.zeros((self.N, self.T), dtype=torch.float32, device=X.device)
+        self._rc(self.lib.ksvd_batch_omp_from_signals_dense(self.h, _p(D), _p(X), _p(G), _p(selected_atoms), _p(c)))
+        return selected_atoms, c
+
+    def BatchOMP_eps(self, D, X, G, eps2, out=None):
+        """Error-constrained Batch-OMP from signals (configured variant)."""
+        idx, val, cnt = self.empty_compact() if out is None else out
+        self._rc(self.lib.ksvd_batch_omp_eps_from_signals(self.h, _p(D), _p(X), _p(G), _p(eps2), _p(idx), _p(val),
+                                                          _p(cnt)))
+        return idx, val, cnt
+
+    def omp_fallbacks(self) -> int:
+        c = C.c_int32()
+        self._rc(self.lib.ksvd_omp_fallbacks(self.h, C.byref(c)))
+        return c.value
 
     def batch_omp_eps(self, DtX, G, xnorm2, eps2, out=None):
         idx, val, cnt = self.empty_compact() if out is None else out
