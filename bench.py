@@ -3,14 +3,17 @@
 Workload (fits one GPU): the precision-form configuration, n=256 signal
 length, K=1024 atoms, N=1,000,000 signals, T=32 atoms per signal, Gaussian
 unit-norm atoms with 25 dB SNR, synthetic and seeded, resident in HBM before
-timing starts.  A step is one Batch-OMP pass over all N signals:
-G = D^T D, DtX = D^T X (MFMA, --form), then the Batch-OMP kernel.
+timing starts.  A step is one Batch-OMP pass over all N signals with the
+headline form: G = D^T D on split-BF16x3 MFMA, then the MFMA-form Batch-OMP
+kernel (correlations D^T r on fp16 MFMA with exact FP32 selection; DtX is
+never materialised).  The plain FP32 form (FP32 GEMMs G and DtX, then the
+wave-per-signal Batch-OMP kernel) is timed beside it.
 
-    python bench.py [--steps 10] [--warmup 2] [--form bf16x3] [--gpus 1]
+    python bench.py [--steps 10] [--warmup 2] [--gpus 1]
 
-Prints ONE JSON line with the metric, roofline of the dominant kernel (the
-Batch-OMP kernel) and the CPU baseline (the FP32 port of the reference
-algorithm in oracle/, timed on a bounded sample of the same signals).
+Prints ONE JSON line: the metric, the roofline of the dominant kernel
+(the MFMA-form Batch-OMP kernel) and the CPU baseline (the FP32 port of the
+reference algorithm in oracle/, timed on a bounded sample of the same signals).
 """
 from __future__ import annotations
 
@@ -24,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Batch-OMP signals/sec + K-SVD iters/sec at n=256,K=1024,T=32; % roofline"
+MFMA_F16_PEAK_TF = 2500.0   # MI355X dense fp16/bf16 MFMA
 FP32_PEAK_TF = 157.3
 HBM_PEAK_GBS = 8000.0
 
@@ -33,7 +37,7 @@ def log(*a):
 
 
 def omp_flops_per_signal(K, T):
-    # alpha update sum_{i<=T} 2 K i + ~3 i^2 per step for the Cholesky row and two solves
+    """Reference algorithm's FP32 work per signal (alpha update + Cholesky/solves)."""
     return K * T * (T + 1) + T * (T + 1) * (2 * T + 1) / 2
 
 
@@ -60,14 +64,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="bf16x3", choices=["fp32", "bf16", "bf16x3"])
+    ap.add_argument("--form", default="bf16x3", choices=["fp32", "bf16", "bf16x3"], help="Gram GEMM form")
+    ap.add_argument("--omp", default="mfma", choices=["mfma", "wave"], help="headline Batch-OMP kernel")
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--ksvd-iters", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain FP32 form (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the engine has no multi-GPU path (see DESIGN.md for how it would shard)",
+        print("bench.py measures one GPU: the engine has no multi-GPU path (DESIGN.md describes how it would shard)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -85,27 +91,23 @@ def main():
     X, D0 = wl.X.contiguous(), wl.D_init.contiguous()
     del wl
     torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
 
-    eng = Engine(n, K, N, T, gemm_form=args.form)
+    eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant=args.omp)
     D = eng.normcols(D0)
     G = torch.empty((K, K), dtype=torch.float32, device=dev)
-    A0 = torch.empty((N, K), dtype=torch.float32, device=dev)
     idx = torch.empty((N, T), dtype=torch.int32, device=dev)
     val = torch.empty((N, T), dtype=torch.float32, device=dev)
     cnt = torch.empty((N,), dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream()
-
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    omp_ms, gemm_ms = [], []
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
 
     def step(record):
         if record:
             ev[0].record(stream)
         eng.gram(D, G)
-        eng.correlate(D, X, A0)
         if record:
             ev[1].record(stream)
-        eng.batch_omp_compact(A0, G, (idx, val, cnt))
+        eng.BatchOMP(D, X, G, (idx, val, cnt))
         if record:
             ev[2].record(stream)
 
@@ -114,20 +116,49 @@ def main():
         step(False)
     torch.cuda.synchronize()
     log("timing", args.steps, "steps")
+    gram_ms, omp_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(True)
-        torch.cuda.synchronize()  # per-step events are read below; sync keeps them valid
-        gemm_ms.append(ev[0].elapsed_time(ev[1]))
+        torch.cuda.synchronize()
+        gram_ms.append(ev[0].elapsed_time(ev[1]))
         omp_ms.append(ev[1].elapsed_time(ev[2]))
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1000.0 / args.steps
     value = N / (ms_per_step / 1000.0)
     omp_avg = float(np.mean(omp_ms))
     c = cnt.cpu().numpy()
-    log(f"step {ms_per_step:.2f} ms (gemms {np.mean(gemm_ms):.2f} ms, omp {omp_avg:.2f} ms), "
-        f"{value / 1e6:.2f} M signals/s, cnt mean {c.mean():.2f}")
+    fallbacks = eng.omp_fallbacks()
+    log(f"step {ms_per_step:.2f} ms (gram {np.mean(gram_ms):.3f} ms, batch-omp {omp_avg:.2f} ms), "
+        f"{value / 1e6:.2f} M signals/s, mean count {c.mean():.2f}, fallbacks {fallbacks}")
+
+    # plain FP32 form: FP32 GEMMs (G, DtX) + wave-per-signal Batch-OMP
+    plain = None
+    if args.plain_steps > 0:
+        eng_p = Engine(n, K, N, T, gemm_form="fp32", omp_variant="wave")
+        A0 = torch.empty((N, K), dtype=torch.float32, device=dev)
+        ip, vp, cp = torch.empty_like(idx), torch.empty_like(val), torch.empty_like(cnt)
+
+        def pstep():
+            eng_p.gram(D, G)
+            eng_p.correlate(D, X, A0)
+            eng_p.batch_omp_compact(A0, G, (ip, vp, cp))
+
+        pstep()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            pstep()
+        torch.cuda.synchronize()
+        p_ms = (time.perf_counter() - t1) * 1000.0 / args.plain_steps
+        same = (cp == cnt) & (ip == idx).all(dim=1)
+        plain = {"ms_per_step": round(p_ms, 3), "signals_per_s": round(N / (p_ms / 1000.0), 1),
+                 "headline_speedup": round(p_ms / ms_per_step, 3),
+                 "identical_supports_vs_headline": round(float(same.float().mean().item()), 6)}
+        log(f"plain FP32 form {p_ms:.2f} ms/step, headline speedup {p_ms / ms_per_step:.2f}x, "
+            f"identical supports {plain['identical_supports_vs_headline']:.4f}")
+        del A0, ip, vp, cp
+        eng_p.close()
 
     # K-SVD iterations (G, Batch-OMP, dictionary update, error, clearDict)
     ksvd_ms = None
@@ -141,18 +172,22 @@ def main():
         ksvd_ms = (time.perf_counter() - t1) * 1000.0 / args.ksvd_iters
         log(f"K-SVD iteration {ksvd_ms:.1f} ms, rmse {rmse}")
 
-    flops = omp_flops_per_signal(K, T) * N
+    # roofline of the dominant kernel: one correlation D^T r (2nK FLOP) per
+    # signal-step; a signal that stopped before T ran one extra step
+    steps_total = float(np.sum(c + (c < T)))
+    flops = 2.0 * n * K * steps_total
     achieved_tf = flops / (omp_avg / 1000.0) / 1e12
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved_tf, 3),
-        "peak": FP32_PEAK_TF,
+        "peak": MFMA_F16_PEAK_TF,
         "unit": "TFLOP/s",
-        "frac": round(achieved_tf / FP32_PEAK_TF, 4),
+        "frac": round(achieved_tf / MFMA_F16_PEAK_TF, 4),
         "traffic": None,
-        "kernel": "omp_wave_kernel<16,false>",
+        "kernel": "omp_fast_kernel<16>" if args.omp == "mfma" else "omp_wave_kernel<16,false>",
         "kernel_ms": round(omp_avg, 3),
-        "work_per_launch": f"{omp_flops_per_signal(K, T):.0f} FP32 FLOP/signal x {N} signals",
+        "work_per_launch": f"2nK = {2 * n * K} FLOP per signal-step x {int(steps_total)} signal-steps",
+        "reference_algorithm_tflops": round(omp_flops_per_signal(K, T) * N / (omp_avg / 1000.0) / 1e12, 3),
     }
 
     cpu = None
@@ -182,10 +217,13 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "C2 precision form: n=256 K=1024 N=1000000 T=32, Gaussian unit atoms, 25 dB SNR",
-                   "n": n, "K": K, "N": N, "T": T, "gemm_form": args.form, "parallelism": "single GPU"},
+                   "n": n, "K": K, "N": N, "T": T, "gram_form": args.form, "batch_omp": args.omp,
+                   "parallelism": "single GPU"},
         "ksvd_iters_per_s": round(1000.0 / ksvd_ms, 3) if ksvd_ms else None,
         "ksvd_ms_per_iter": round(ksvd_ms, 2) if ksvd_ms else None,
-        "phase_ms": {"gemms": round(float(np.mean(gemm_ms)), 3), "batch_omp": round(omp_avg, 3)},
+        "phase_ms": {"gram": round(float(np.mean(gram_ms)), 3), "batch_omp": round(omp_avg, 3)},
+        "omp_fallbacks": fallbacks,
+        "plain_fp32_form": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

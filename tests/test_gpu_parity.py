@@ -237,7 +237,7 @@ def test_batch_omp_eps_parity(o64):
             near += 1
     say(f"eps supports: equal {eq}, ties {tie}, off-by-one stops {near}, other {mis - near}")
     assert mis - near == 0 and near <= 0.01 * N
-    assert np.all(cnt_g <= Tmax) and cnt_g.min() >= 1
+    assert np.all(cnt_g <= Tmax) and cnt_g.min() >= 0  # cnt 0: ||x||^2 <= eps^2 already
     eng.close()
 
 
@@ -318,9 +318,11 @@ def test_error_and_clear_dict_parity(o64):
     r_ref, e_ref = o64.error(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt)
     assert abs(rmse - r_ref) <= 1e-6 * r_ref
     np.testing.assert_allclose(host(err), e_ref, rtol=1e-4, atol=1e-6)
+    err_before = host(err).astype(np.float64)  # clearDict zeroes the picked entries in place
     eng.clearDict(Xg, Dg, idg, vg, cg, err)
     Dr, er, flags = o64.clear_dict(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt,
-                                   np.zeros(K, np.int32), host(err).astype(np.float64))
+                                   np.zeros(K, np.int32), err_before)
+    np.testing.assert_array_equal(host(err) == 0, er == 0)
     assert flags[0] == 1
     np.testing.assert_allclose(host(Dg), Dr, atol=1e-5)
     eng.close()
@@ -436,8 +438,9 @@ def test_c2_full_size_properties(o64):
     idx, val, cnt = eng.batch_omp_compact(A0, G)
     torch.cuda.synchronize()
     c = cnt.cpu().numpy()
-    # the K signals that became atoms stop after one atom; the rest take T
-    assert np.all((c == T) | (c == 1)) and np.sum(c == 1) <= K
+    # only the K signals that became atoms can stop before T (exact fit)
+    say("count histogram (count: signals)", dict(zip(*np.unique(c, return_counts=True))))
+    assert np.all((c >= 1) & (c <= T)) and np.sum(c < T) <= K
     ii = idx.cpu().numpy()
     assert np.all((ii[:, 0] >= 0) & (ii[:, 0] < K))
     rng = np.random.default_rng(0)
