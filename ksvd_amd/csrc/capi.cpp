@@ -119,20 +119,18 @@ int ensure_du(ksvd_handle* h) {
     w.mu_thresh = c.mu_thresh;
     w.chunk = 256;
     w.nchunks = (c.N + w.chunk - 1) / w.chunk;
-    w.vparts = 64;
     w.rmse_parts = 2048;
-    const int M = c.N < c.max_signals ? c.N : c.max_signals;
     TRY(alloc(h, &w.count, c.K, "count"));
     TRY(alloc(h, &w.ptr, c.K + 1, "ptr"));
     TRY(alloc(h, &w.chunk_hist, (size_t)c.K * w.nchunks, "chunk_hist"));
     TRY(alloc(h, &w.list_pos, (size_t)c.N * c.T, "list_pos"));
-    TRY(alloc(h, &w.gn2, c.K, "gn2"));
-    TRY(alloc(h, &w.Y, (size_t)c.n * c.K, "Y"));
-    TRY(alloc(h, &w.vpart, (size_t)w.vparts * c.K, "vpart"));
-    TRY(alloc(h, &w.avec, c.n, "avec"));
-    TRY(alloc(h, &w.uvec, c.K, "uvec"));
+    TRY(alloc(h, &w.R, (size_t)c.n * c.N, "R"));
+    TRY(alloc(h, &w.apart, (size_t)256 * c.n, "apart"));
+    TRY(alloc(h, &w.bpart, (size_t)2 * 256, "bpart"));
+    TRY(alloc(h, &w.dnew, c.n, "dnew"));
     TRY(alloc(h, &w.scal, 8, "scal"));
-    TRY(alloc(h, &w.err_cand, M, "err_cand"));
+    TRY(alloc(h, &w.tickets, 4, "tickets"));
+    HIPTRY(h, hipMemsetAsync(w.tickets, 0, 4 * sizeof(int32_t), h->stream), "tickets");
     TRY(alloc(h, &w.unused, c.N, "unused"));
     TRY(alloc(h, &w.us_counter, 1, "us_counter"));
     TRY(alloc(h, &w.bitmap, c.N, "bitmap"));

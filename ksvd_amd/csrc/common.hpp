@@ -130,13 +130,12 @@ struct DuWork {
     int32_t* ptr;        // K + 1
     int32_t* chunk_hist; // K x nchunks
     int32_t* list_pos;   // N*T
-    float* gn2;          // K
-    float* Y;            // n x K
-    float* vpart;        // vparts x K
-    float* avec;         // n
-    float* uvec;         // K
+    float* R;            // n x N restricted residuals X - D Gamma
+    float* apart;        // DU_BLOCKS x n partial a
+    float* bpart;        // DU_BLOCKS x 2 partial arg-max (special case)
+    float* dnew;         // n
     float* scal;         // 8 scalars (norm etc.)
-    float* err_cand;     // max_signals
+    int32_t* tickets;    // 4 last-block-done counters (zero between launches)
     int32_t* unused;     // N
     int32_t* us_counter; // 1
     int32_t* bitmap;     // N
@@ -148,7 +147,6 @@ struct DuWork {
     double* rmse_dev;    // 1
     int32_t* repl_list;  // K (clearDict replaced-in-sweep list) + 1 count
     int nchunks, chunk;
-    int vparts;
     int rmse_parts;
 };
 hipError_t launch_du_init(DuWork& w, hipStream_t st);

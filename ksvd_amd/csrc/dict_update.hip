@@ -23,7 +23,8 @@ namespace ksvd {
 
 namespace {
 
-constexpr int UPD_BLOCKS = 2048;  // grid of the per-entry update kernel (grid-stride)
+constexpr int DU_BLOCKS = 256;     // grid of the per-atom phase kernels (grid-stride over the atom's list)
+constexpr int RES_BLOCKS = 2048;   // grid of the residual initialisation
 
 __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -145,59 +146,7 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
     }
 }
 
-// ---- precompute ||gamma_k||^2 and Y_k = X(:, I_k) gamma_k -----------------
-// S/ReductionKernels.cu:31-98 (STAGE1 + STAGE2_together_collincomb)
-__global__ __launch_bounds__(256) void du_prep_kernel(const float* __restrict__ X, const float* __restrict__ val,
-                                                      const int32_t* __restrict__ list_pos,
-                                                      const int32_t* __restrict__ ptr, int n, int T,
-                                                      float* __restrict__ gn2, float* __restrict__ Y) {
-    __shared__ float red[1024];
-    __shared__ float wred[16];
-    const int k = blockIdx.x;
-    const int beg = ptr[k], end = ptr[k + 1];
-    float sq = 0.f;
-    for (int m = beg + threadIdx.x; m < end; m += blockDim.x) {
-        const float v = val[list_pos[m]];
-        sq += v * v;
-    }
-    sq = block_sum(sq, wred);
-    if (threadIdx.x == 0) gn2[k] = sq;
-
-    const int npad = min(256, ((n + 63) / 64) * 64);
-    const int ng = 256 / npad;
-    const int g = threadIdx.x / npad, tr = threadIdx.x % npad;
-    for (int t0 = 0; t0 < n; t0 += npad) {
-        const int t = t0 + tr;
-        float acc = 0.f;
-        if (t < n) {
-            int m = beg + g;
-            for (; m + 3 * ng < end; m += 4 * ng) {
-                const int p0 = list_pos[m], p1 = list_pos[m + ng], p2 = list_pos[m + 2 * ng],
-                          p3 = list_pos[m + 3 * ng];
-                const float x0 = X[(long)(p0 / T) * n + t], x1 = X[(long)(p1 / T) * n + t];
-                const float x2 = X[(long)(p2 / T) * n + t], x3 = X[(long)(p3 / T) * n + t];
-                acc += x0 * val[p0];
-                acc += x1 * val[p1];
-                acc += x2 * val[p2];
-                acc += x3 * val[p3];
-            }
-            for (; m < end; m += ng) {
-                const int p0 = list_pos[m];
-                acc += X[(long)(p0 / T) * n + t] * val[p0];
-            }
-        }
-        __syncthreads();
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        if (g == 0 && t < n) {
-            float s = 0.f;
-            for (int q = 0; q < ng; ++q) s += red[q * npad + tr];
-            Y[(long)k * n + t] = s;
-        }
-    }
-}
-
-// ---- per-atom step kernels -------------------------------------------------
+// ---- restricted residuals -------------------------------------------------
 // Sparse reconstruction error of one signal by one wave: ||x - D gamma_s||^2.
 __device__ __forceinline__ float wave_signal_error(const float* __restrict__ X, const float* __restrict__ D,
                                                    const int32_t* __restrict__ idx,
@@ -222,215 +171,241 @@ __device__ __forceinline__ float wave_signal_error(const float* __restrict__ X, 
     return wave_sum(e);
 }
 
-// S/ReductionKernels.cu:436-456 (SCase_err_stage1), without the dense D*Gamma.
-__global__ __launch_bounds__(256) void du_special_err_kernel(int jj, const uint32_t* __restrict__ perm,
-                                                             const int32_t* __restrict__ count,
-                                                             const float* __restrict__ X,
-                                                             const float* __restrict__ D,
-                                                             const int32_t* __restrict__ idx,
-                                                             const float* __restrict__ val,
-                                                             const int32_t* __restrict__ cnt,
-                                                             const int32_t* __restrict__ unused,
-                                                             const int32_t* __restrict__ usc, int n, int N, int T,
-                                                             int M, float* __restrict__ err_cand) {
-    const int p = (int)perm[jj];
-    if (count[p] != 0) return;
-    const int live = N - *usc;
-    const int lim = min(M, live);
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= lim) return;
+// R = X - D Gamma, one wave per signal (grid-stride).
+__global__ __launch_bounds__(256) void du_residual_kernel(const float* __restrict__ X, const float* __restrict__ D,
+                                                          const int32_t* __restrict__ idx,
+                                                          const float* __restrict__ val,
+                                                          const int32_t* __restrict__ cnt, int n, int N, int T,
+                                                          float* __restrict__ R) {
     const int lane = threadIdx.x & 63;
-    const long sig = unused[b];
-    const float e = wave_signal_error(X, D, idx, val, cnt[sig], sig, n, T, lane);
-    if (lane == 0) err_cand[b] = e;
+    for (long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6); s < N; s += (long)gridDim.x * 4) {
+        const int c = cnt[s];
+        const int il = (lane < c) ? idx[s * T + lane] : 0;
+        const float vl = (lane < c) ? val[s * T + lane] : 0.f;
+        for (int t0 = 0; t0 < n; t0 += 64) {
+            const int t = t0 + lane;
+            float xh = 0.f;
+            for (int j = 0; j < c; ++j) {
+                const int col = readlane(il, j);
+                const float g = readlane(vl, j);
+                if (t < n) xh += D[(long)col * n + t] * g;
+            }
+            if (t < n) R[s * n + t] = X[s * n + t] - xh;
+        }
+    }
 }
 
-// S/ReductionKernels.cu:462-539 + SCupdate S/DictionaryUpdate.cu:406-427.
-__global__ __launch_bounds__(1024) void du_special_apply_kernel(int jj, const uint32_t* __restrict__ perm,
-                                                                const int32_t* __restrict__ count,
-                                                                const float* __restrict__ X, float* __restrict__ D,
-                                                                int32_t* __restrict__ unused,
-                                                                int32_t* __restrict__ usc,
-                                                                int32_t* __restrict__ bitmap,
-                                                                int32_t* __restrict__ replaced, int n, int N, int M,
-                                                                const float* __restrict__ err_cand) {
-    __shared__ float rv[16];
-    __shared__ int ri[16];
-    __shared__ float wred[16];
-    const int p = (int)perm[jj];
-    if (count[p] != 0) return;
-    const int counter = *usc;
-    const int lim = min(M, N - counter);
-    float bv = 0.f;  // reference seeds lanes with value 0 (S/ReductionKernels.cu:468-476)
-    int bi = 0;
-    for (int b = threadIdx.x; b < lim; b += blockDim.x) {
-        const float v = err_cand[b];
-        if (v > bv) { bv = v; bi = b; }
-    }
-    wave_argmax(bv, bi);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { rv[w] = bv; ri[w] = bi; }
+// Last-block-done: returns true in exactly one block, after every block of
+// the grid has made its global writes visible.  The last block re-arms the
+// ticket, so the kernel can be launched again (and replayed in a graph).
+__device__ __forceinline__ bool last_block(int32_t* ticket, int* flag_sh) {
+    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        float v = rv[0];
-        int i = ri[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
-            if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
-        ri[0] = i;
+        const int old = atomicAdd(ticket, 1);
+        *flag_sh = (old == (int)gridDim.x - 1);
     }
     __syncthreads();
-    const int best = ri[0];
-    const long sig = unused[best];
+    const bool last = *flag_sh != 0;
+    if (last) {
+        __threadfence();
+        if (threadIdx.x == 0) *ticket = 0;
+    }
+    return last;
+}
+
+// Per-atom phase A.
+//   N_p > 0:  a = sum_{s in I_p} gamma_s (r_s + d_p gamma_s)  ( = E_p gamma^T,
+//             S/Multiplication.cu:177-193 computes the same vector as
+//             Y_p - D v + d_p ||gamma||^2 );  the last block forms ||a|| and
+//             d_new = a / ||a|| (S/Multiplication.cu:221).
+//   N_p = 0:  the unused-atom special case (S/DictionaryUpdate.cu:238-302):
+//             errors ||r_b||^2 = ||x - D Gamma||^2 of the first
+//             min(N, max_signals) live unused-list entries, arg-max (lowest
+//             index on ties), d_p = x / ||x||, swap-remove, replaced, bitmap
+//             (SCupdate, S/DictionaryUpdate.cu:406-427).
+__global__ __launch_bounds__(256) void du_phase_a_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                         const int32_t* __restrict__ count,
+                                                         const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ list_pos,
+                                                         const float* __restrict__ val, const float* __restrict__ X,
+                                                         float* __restrict__ D, const float* __restrict__ R,
+                                                         int32_t* __restrict__ unused, int32_t* __restrict__ usc,
+                                                         int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced,
+                                                         int n, int N, int T, int M, float* __restrict__ apart,
+                                                         float* __restrict__ bpart, float* __restrict__ dnew,
+                                                         float* __restrict__ scal, int32_t* __restrict__ ticket) {
+    extern __shared__ float sh[];  // 4 x n (per-wave partial a)
+    __shared__ float rv[4];
+    __shared__ int ri[4];
+    __shared__ int flag_sh;
+    __shared__ float wred[16];
+    const int p = (int)perm[jj];
+    const int Np = count[p];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    if (Np == 0) {
+        const int counter = *usc;
+        const int lim = min(M, N - counter);
+        float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
+        int bi = 0;
+        for (int b = blockIdx.x * 4 + w; b < lim; b += G * 4) {
+            const long sig = unused[b];
+            float e = 0.f;
+            for (int t = lane; t < n; t += 64) {
+                const float r = R[sig * n + t];
+                e += r * r;
+            }
+            e = wave_sum(e);
+            if (e > bv) { bv = e; bi = b; }  // b ascending per wave: keeps the lowest on ties
+        }
+        if (lane == 0) { rv[w] = bv; ri[w] = bi; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float v = rv[0];
+            int i = ri[0];
+            for (int q = 1; q < 4; ++q)
+                if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
+            bpart[2 * blockIdx.x] = v;
+            bpart[2 * blockIdx.x + 1] = __int_as_float(i);
+        }
+        if (!last_block(ticket, &flag_sh)) return;
+        if (threadIdx.x == 0) {
+            float v = bpart[0];
+            int i = __float_as_int(bpart[1]);
+            for (int q = 1; q < G; ++q) {
+                const float vq = bpart[2 * q];
+                const int iq = __float_as_int(bpart[2 * q + 1]);
+                if (vq > v || (vq == v && iq < i)) { v = vq; i = iq; }
+            }
+            ri[0] = i;
+        }
+        __syncthreads();
+        const int best = ri[0];
+        const long sig = unused[best];
+        float sq = 0.f;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const float x = X[sig * n + t];
+            sq += x * x;
+        }
+        sq = block_sum(sq, wred);
+        const float nrm = sqrtf(sq);
+        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unused[best] = unused[N - counter - 1];
+            *usc = counter + 1;
+            replaced[p] = 1;
+            bitmap[sig] = 0;
+        }
+        return;
+    }
+    const int beg = ptr[p];
+    const float* dp = D + (long)p * n;
+    for (int t0 = 0; t0 < n; t0 += 256) {
+        // lanes cover t0 + lane + 64 q, q < 4
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        float dv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = t0 + lane + 64 * q;
+            dv[q] = (t < n) ? dp[t] : 0.f;
+        }
+        for (int m = blockIdx.x * 4 + w; m < Np; m += G * 4) {
+            const int pos = list_pos[beg + m];
+            const long s = pos / T;
+            const float g = val[pos];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = t0 + lane + 64 * q;
+                if (t < n) acc[q] += g * (R[s * n + t] + dv[q] * g);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = t0 + lane + 64 * q;
+            if (t < n) sh[w * n + t] = acc[q];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+        apart[(long)blockIdx.x * n + t] = ((sh[t] + sh[n + t]) + sh[2 * n + t]) + sh[3 * n + t];
+    if (!last_block(ticket, &flag_sh)) return;
     float sq = 0.f;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const float x = X[sig * n + t];
-        sq += x * x;
+        float a = 0.f;
+        for (int b = 0; b < G; ++b) a += apart[(long)b * n + t];
+        sh[t] = a;
+        sq += a * a;
     }
     sq = block_sum(sq, wred);
     const float nrm = sqrtf(sq);
-    for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unused[best] = unused[N - counter - 1];
-        *usc = counter + 1;
-        replaced[p] = 1;
-        bitmap[sig] = 0;
-    }
+    for (int t = threadIdx.x; t < n; t += blockDim.x) dnew[t] = sh[t] / nrm;
+    if (threadIdx.x == 0) scal[0] = nrm;
 }
 
-// v = Gamma(:, I_p) gamma_p^T over rows != p  (S/ReductionKernels.cu:105-139);
-// one thread per list entry, LDS float atomics, one partial vector per block.
-__global__ __launch_bounds__(256) void du_v_kernel(int jj, const uint32_t* __restrict__ perm,
-                                                   const int32_t* __restrict__ count,
-                                                   const int32_t* __restrict__ ptr,
-                                                   const int32_t* __restrict__ list_pos,
-                                                   const int32_t* __restrict__ idx, const float* __restrict__ val,
-                                                   const int32_t* __restrict__ cnt, int K, int T, int vparts,
-                                                   float* __restrict__ vpart) {
-    extern __shared__ float v[];
+// Per-atom phase B: gamma_new = d_new^T (r_s + d_p gamma_s) and the residual
+// update r_s <- r_s + d_p gamma_s - d_new gamma_new for every signal on the
+// list (S/Multiplication.cu:198-225 evaluates the same gamma_new as
+// (a^T x - u^T Gamma_s + u_p gamma_s) / ||a||); the last block writes
+// d_p = d_new (S/Multiplication.cu:221).
+__global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                         const int32_t* __restrict__ count,
+                                                         const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ list_pos, float* __restrict__ val,
+                                                         float* __restrict__ D, float* __restrict__ R,
+                                                         const float* __restrict__ dnew, int n, int T,
+                                                         int32_t* __restrict__ ticket) {
+    __shared__ int flag_sh;
     const int p = (int)perm[jj];
     const int Np = count[p];
     if (Np == 0) return;
-    for (int k = threadIdx.x; k < K; k += blockDim.x) v[k] = 0.f;
-    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int G = gridDim.x;
     const int beg = ptr[p];
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < Np; m += vparts * blockDim.x) {
-        const int pos = list_pos[beg + m];
-        const long s = pos / T;
-        const float gp = val[pos];
-        const int c = cnt[s];
-        const int32_t* ip = idx + s * T;
-        const float* vp = val + s * T;
-        for (int j = 0; j < c; ++j) {
-            const int r = ip[j];
-            if (r != p) atomicAdd(&v[r], vp[j] * gp);
+    const float* dp = D + (long)p * n;
+    if (n <= 256) {
+        float dv[4], dn[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = lane + 64 * q;
+            dv[q] = (t < n) ? dp[t] : 0.f;
+            dn[q] = (t < n) ? dnew[t] : 0.f;
+        }
+        for (int m = blockIdx.x * 4 + w; m < Np; m += G * 4) {
+            const int pos = list_pos[beg + m];
+            const long s = pos / T;
+            const float g = val[pos];
+            float e[4];
+            float dot = 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = lane + 64 * q;
+                e[q] = (t < n) ? R[s * n + t] + dv[q] * g : 0.f;
+                dot += dn[q] * e[q];
+            }
+            const float gn = wave_sum(dot);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = lane + 64 * q;
+                if (t < n) R[s * n + t] = e[q] - dn[q] * gn;
+            }
+            if (lane == 0) val[pos] = gn;
+        }
+    } else {
+        for (int m = blockIdx.x * 4 + w; m < Np; m += G * 4) {
+            const int pos = list_pos[beg + m];
+            const long s = pos / T;
+            const float g = val[pos];
+            float dot = 0.f;
+            for (int t = lane; t < n; t += 64) dot += dnew[t] * (R[s * n + t] + dp[t] * g);
+            const float gn = wave_sum(dot);
+            for (int t = lane; t < n; t += 64) R[s * n + t] = R[s * n + t] + dp[t] * g - dnew[t] * gn;
+            if (lane == 0) val[pos] = gn;
         }
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < K; k += blockDim.x) vpart[(long)blockIdx.x * K + k] = v[k];
-}
-
-// a = Y_p - D v + d_p ||gamma_p||^2   (device_A_times_x_modified, S/Multiplication.cu:177-193)
-// ||a|| and u = D^T a                  (rowlincomb, S/ReductionKernels.cu:196-250)
-__global__ __launch_bounds__(1024) void du_atom_kernel(int jj, const uint32_t* __restrict__ perm,
-                                                       const int32_t* __restrict__ count,
-                                                       const float* __restrict__ vpart, int vparts,
-                                                       const float* __restrict__ gn2, const float* __restrict__ Y,
-                                                       const float* __restrict__ D, int n, int K,
-                                                       float* __restrict__ avec, float* __restrict__ uvec,
-                                                       float* __restrict__ scal) {
-    extern __shared__ float sh[];
-    float* v = sh;          // K
-    float* a = sh + K;      // n
-    float* red = a + n;     // 1024
-    __shared__ float wred[16];
-    const int p = (int)perm[jj];
-    if (count[p] == 0) return;
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
-        float s = 0.f;
-        for (int b = 0; b < vparts; ++b) s += vpart[(long)b * K + k];
-        v[k] = s;
-    }
-    __syncthreads();
-    const float g2 = gn2[p];
-    if (threadIdx.x == 0) v[p] = g2;
-    __syncthreads();
-    // D v: thread (g, tr) sums columns r = g, g+ng, ... for row t
-    const int npad = min(1024, ((n + 63) / 64) * 64);
-    const int ng = 1024 / npad;
-    const int g = threadIdx.x / npad, tr = threadIdx.x % npad;
-    for (int t0 = 0; t0 < n; t0 += npad) {
-        const int t = t0 + tr;
-        float acc = 0.f;
-        if (t < n)
-            for (int r = g; r < K; r += ng) acc += D[(long)r * n + t] * v[r];
-        __syncthreads();
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        if (g == 0 && t < n) {
-            float s = 0.f;
-            for (int q = 0; q < ng; ++q) s += red[q * npad + tr];
-            a[t] = Y[(long)p * n + t] - s + D[(long)p * n + t] * g2;
-        }
-    }
-    __syncthreads();
-    float sq = 0.f;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        sq += a[t] * a[t];
-        avec[t] = a[t];
-    }
-    sq = block_sum(sq, wred);
-    if (threadIdx.x == 0) scal[0] = sqrtf(sq);
-    // u = D^T a: one wave per column
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int r = w; r < K; r += nw) {
-        float s = 0.f;
-        for (int t = lane; t < n; t += 64) s += D[(long)r * n + t] * a[t];
-        s = wave_sum(s);
-        if (lane == 0) uvec[r] = s;
-    }
-}
-
-// gamma_new (device_normal_x_times_A_modified, S/Multiplication.cu:198-225):
-// one wave per list entry; the slots of signal s are held one per lane.
-__global__ __launch_bounds__(256) void du_update_kernel(int jj, const uint32_t* __restrict__ perm,
-                                                        const int32_t* __restrict__ count,
-                                                        const int32_t* __restrict__ ptr,
-                                                        const int32_t* __restrict__ list_pos,
-                                                        const int32_t* __restrict__ idx, float* __restrict__ val,
-                                                        const int32_t* __restrict__ cnt,
-                                                        const float* __restrict__ X, float* __restrict__ D,
-                                                        const float* __restrict__ avec,
-                                                        const float* __restrict__ uvec,
-                                                        const float* __restrict__ scal, int n, int T) {
-    extern __shared__ float a[];
-    const int p = (int)perm[jj];
-    const int Np = count[p];
-    if (Np == 0) return;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) a[t] = avec[t];
-    __syncthreads();
-    const float nrm = scal[0];
-    const float up = uvec[p];
-    const int lane = threadIdx.x & 63;
-    const int beg = ptr[p];
-    const int wg = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int nwg = gridDim.x * 4;
-    for (int m = wg; m < Np; m += nwg) {
-        const int pos = list_pos[beg + m];
-        const long s = pos / T;
-        const int c = cnt[s];
-        float wdot = 0.f;
-        for (int t = lane; t < n; t += 64) wdot += X[s * n + t] * a[t];
-        const float su = (lane < c) ? val[s * T + lane] * uvec[idx[s * T + lane]] : 0.f;
-        wdot = wave_sum(wdot);
-        const float sum = wave_sum(su);
-        if (lane == 0) {
-            const float gold = val[pos];
-            val[pos] = (wdot - sum + up * gold) / nrm;
-        }
-    }
-    if (blockIdx.x == 0)
-        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = a[t] / nrm;
+    if (!last_block(ticket, &flag_sh)) return;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = dnew[t];
 }
 
 // ---- error: S/KSVD.cu:235-262, S/ReductionKernels.cu:257-322 ---------------
@@ -640,21 +615,15 @@ hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* i
                            hipStream_t st) {
     hipError_t e = launch_du_build_lists(w, idx, cnt, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(du_prep_kernel, dim3(w.K), dim3(256), 0, st, X, val, w.list_pos, w.ptr, w.n, w.T, w.gn2, w.Y);
+    hipLaunchKernelGGL(du_residual_kernel, dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n, w.N, w.T, w.R);
     const int M = w.N < w.max_signals ? w.N : w.max_signals;
-    const unsigned spec_blocks = blocks_for(M, 4);
-    const size_t atom_lds = ((size_t)w.K + w.n + 1024) * 4;
+    const size_t lds_a = (size_t)4 * w.n * sizeof(float);
     for (int jj = 0; jj < w.K; ++jj) {
-        hipLaunchKernelGGL(du_special_err_kernel, dim3(spec_blocks), dim3(256), 0, st, jj, w.perm_dev, w.count, X, D,
-                           idx, val, cnt, w.unused, w.us_counter, w.n, w.N, w.T, M, w.err_cand);
-        hipLaunchKernelGGL(du_special_apply_kernel, dim3(1), dim3(1024), 0, st, jj, w.perm_dev, w.count, X, D,
-                           w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, M, w.err_cand);
-        hipLaunchKernelGGL(du_v_kernel, dim3(w.vparts), dim3(256), (size_t)w.K * 4, st, jj, w.perm_dev, w.count,
-                           w.ptr, w.list_pos, idx, val, cnt, w.K, w.T, w.vparts, w.vpart);
-        hipLaunchKernelGGL(du_atom_kernel, dim3(1), dim3(1024), atom_lds, st, jj, w.perm_dev, w.count, w.vpart,
-                           w.vparts, w.gn2, w.Y, D, w.n, w.K, w.avec, w.uvec, w.scal);
-        hipLaunchKernelGGL(du_update_kernel, dim3(UPD_BLOCKS), dim3(256), (size_t)w.n * 4, st, jj, w.perm_dev,
-                           w.count, w.ptr, w.list_pos, idx, val, cnt, X, D, w.avec, w.uvec, w.scal, w.n, w.T);
+        hipLaunchKernelGGL(du_phase_a_kernel, dim3(DU_BLOCKS), dim3(256), lds_a, st, jj, w.perm_dev, w.count, w.ptr,
+                           w.list_pos, val, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T, M,
+                           w.apart, w.bpart, w.dnew, w.scal, w.tickets);
+        hipLaunchKernelGGL(du_phase_b_kernel, dim3(DU_BLOCKS), dim3(256), 0, st, jj, w.perm_dev, w.count, w.ptr,
+                           w.list_pos, val, D, w.R, w.dnew, w.n, w.T, w.tickets + 1);
     }
     return hipGetLastError();
 }
