@@ -52,14 +52,18 @@ enum {
     KSVD_FORM_BF16X3 = 2  /* split-BF16 x3: hi*hi + hi*lo + lo*hi, FP32 acc */
 };
 
-/* Batch-OMP kernel variant (entry points that receive D and X). */
+/* Batch-OMP kernel variant. */
 enum {
-    KSVD_OMP_AUTO = 0,  /* MFMA form when n <= 256, else the wave form          */
-    KSVD_OMP_WAVE = 1,  /* one wave64 per signal: alpha = alpha0 - G(:,I) gamma  */
-                        /* in VGPRs over coalesced G columns, L in LDS           */
-    KSVD_OMP_MFMA = 2   /* 64-signal tiles: alpha = D^T r on fp16 MFMA with a    */
-                        /* rigorous error bound, exact FP32 re-evaluation of the */
-                        /* candidates within it; n <= 256                        */
+    KSVD_OMP_AUTO = 0,   /* compressed-G wave form when K % 4 == 0 and            */
+                         /* 256 <= K <= 1024, else the FP32 wave form             */
+    KSVD_OMP_WAVE = 1,   /* one wave64 per signal: alpha = alpha0 - G(:,I) gamma   */
+                         /* from FP32 columns of G in VGPRs, L in LDS             */
+    KSVD_OMP_MFMA = 2,   /* 64-signal tiles: alpha = D^T r on fp16 MFMA with a     */
+                         /* rigorous error bound, exact FP32 re-evaluation of the  */
+                         /* candidates; entry points taking D and X, n <= 256     */
+    KSVD_OMP_WAVE16 = 3  /* wave form streaming an fp16 copy of G, rigorous error  */
+                         /* bound per atom, exact FP32 re-evaluation of the        */
+                         /* candidates within it (selection = FP32 arg-max)        */
 };
 
 /*
@@ -150,9 +154,10 @@ int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float*
                                     const float* G, const float* eps2, int32_t* idx, float* val,
                                     int32_t* cnt);
 
-/* Number of signal-steps of the last MFMA-form call whose candidate lists were
- * incomplete and fell back to FP32 correlations with every atom (synchronises). */
-int ksvd_omp_fallbacks(ksvd_handle* h, int32_t* count);
+/* Diagnostics of the last Batch-OMP call (synchronises): candidates
+ * re-evaluated in FP32 and steps that fell back to the full FP32 update
+ * (compressed-G form), or fallbacks of the MFMA form.  NULL outputs allowed. */
+int ksvd_omp_stats(ksvd_handle* h, int64_t* candidates, int64_t* fallbacks);
 
 /* ---- dictionary update / error / clear ------------------------------------ */
 

@@ -12,8 +12,8 @@ LIB_PATH = os.path.join(HERE, "libksvd_amd.so")
 
 FORM_FP32, FORM_BF16, FORM_BF16X3 = 0, 1, 2
 FORMS = {"fp32": FORM_FP32, "bf16": FORM_BF16, "bf16x3": FORM_BF16X3}
-OMP_AUTO, OMP_WAVE, OMP_MFMA = 0, 1, 2
-OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "mfma": OMP_MFMA}
+OMP_AUTO, OMP_WAVE, OMP_MFMA, OMP_WAVE16 = 0, 1, 2, 3
+OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "mfma": OMP_MFMA, "wave16": OMP_WAVE16}
 
 KSVD_OK = 0
 STATUS = {0: "KSVD_OK", -1: "KSVD_ERR_INVALID", -2: "KSVD_ERR_LIMIT", -3: "KSVD_ERR_HIP", -4: "KSVD_ERR_ALLOC",
@@ -24,7 +24,7 @@ EXPORTS = [
     "ksvd_config_default", "ksvd_create", "ksvd_destroy", "ksvd_set_stream", "ksvd_last_error", "ksvd_version",
     "ksvd_sync", "ksvd_normcols", "ksvd_at_times_b", "ksvd_gram", "ksvd_correlate", "ksvd_batch_omp",
     "ksvd_batch_omp_compact", "ksvd_batch_omp_from_signals", "ksvd_batch_omp_from_signals_dense",
-    "ksvd_batch_omp_eps", "ksvd_batch_omp_eps_from_signals", "ksvd_omp_fallbacks", "ksvd_du_init",
+    "ksvd_batch_omp_eps", "ksvd_batch_omp_eps_from_signals", "ksvd_omp_stats", "ksvd_du_init",
     "ksvd_dictionary_update", "ksvd_dictionary_update_dense", "ksvd_error", "ksvd_clear_dict",
     "ksvd_execute_iterations", "ksvd_perm_sequence", "ksvd_omp_adjoint",
 ]
@@ -79,7 +79,7 @@ def load() -> C.CDLL:
     sig("ksvd_batch_omp_from_signals_dense", I, H, P, P, P, P, P)
     sig("ksvd_batch_omp_eps", I, H, P, P, P, P, P, P, P)
     sig("ksvd_batch_omp_eps_from_signals", I, H, P, P, P, P, P, P, P)
-    sig("ksvd_omp_fallbacks", I, H, C.POINTER(C.c_int32))
+    sig("ksvd_omp_stats", I, H, C.POINTER(C.c_int64), C.POINTER(C.c_int64))
     sig("ksvd_du_init", I, H)
     sig("ksvd_dictionary_update", I, H, P, P, P, P, P, P)
     sig("ksvd_dictionary_update_dense", I, H, P, P, P, P, P)

@@ -39,6 +39,10 @@ struct ksvd_handle {
     void* fast = nullptr;      // MFMA-form Batch-OMP scratch
     int fast_grid = 0;
     float* xnorm2 = nullptr;   // N (wave-form error mode from signals)
+    _Float16* gh = nullptr;    // compressed-G form: fp16 G
+    float* gbound = nullptr;   // compressed-G form: per-atom bounds
+    unsigned long long* omp_stats = nullptr;  // [0] candidates, [1] fallbacks
+    int last_variant = 0;
 };
 
 // ||x_s||^2 per signal (one wave per signal).
@@ -103,7 +107,7 @@ int check_cfg(ksvd_handle* h, const ksvd_config* c) {
     if (c->n > 4096) return fail(h, KSVD_ERR_LIMIT, "n=%d exceeds 4096", c->n);
     if ((long)c->N * c->T > 0x7fffffffL) return fail(h, KSVD_ERR_LIMIT, "N*T exceeds 2^31");
     if (c->gemm_form < 0 || c->gemm_form > 2) return fail(h, KSVD_ERR_INVALID, "gemm_form=%d", c->gemm_form);
-    if (c->omp_variant < 0 || c->omp_variant > 2) return fail(h, KSVD_ERR_INVALID, "omp_variant=%d", c->omp_variant);
+    if (c->omp_variant < 0 || c->omp_variant > 3) return fail(h, KSVD_ERR_INVALID, "omp_variant=%d", c->omp_variant);
     if (c->mu_thresh < 0) return fail(h, KSVD_ERR_INVALID, "mu_thresh must be >= 0");
     if (c->max_signals < 1) return fail(h, KSVD_ERR_INVALID, "max_signals must be >= 1");
     return KSVD_OK;
@@ -178,10 +182,35 @@ ksvd::OmpArgs omp_args(ksvd_handle* h, const float* DtX, const float* G) {
     return a;
 }
 
-bool use_fast(const ksvd_handle* h) {
+bool use_fast(const ksvd_handle* h) { return h->cfg.omp_variant == KSVD_OMP_MFMA && h->cfg.n <= 256; }
+
+bool use_g16(const ksvd_handle* h) {
     const int v = h->cfg.omp_variant;
-    if (v == KSVD_OMP_WAVE) return false;
-    return h->cfg.n <= 256;
+    if (v == KSVD_OMP_WAVE16) return ksvd::g16_supported(h->cfg.K);
+    return v == KSVD_OMP_AUTO && ksvd::g16_supported(h->cfg.K) && h->cfg.K >= 256;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// Fills the wave-form arguments' compressed-G fields when that form is in use.
+int prepare_wave(ksvd_handle* h, const float* G, ksvd::OmpArgs& a) {
+    const ksvd_config& c = h->cfg;
+    TRY(alloc(h, &h->omp_stats, 2, "omp stats"));
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, h->stream, h->omp_stats, 2);
+    a.stats = h->omp_stats;
+    h->last_variant = KSVD_OMP_WAVE;
+    if (!use_g16(h)) return KSVD_OK;
+    const int Kpad = ksvd::g16_kpad(c.K);
+    TRY(alloc(h, &h->gh, (size_t)c.K * Kpad, "Gh"));
+    TRY(alloc(h, &h->gbound, (size_t)2 * c.K, "gbound"));
+    HIPTRY(h, ksvd::launch_g16_prep(G, c.K, h->gh, h->gbound, h->stream), "g16 prep");
+    a.Gh = h->gh;
+    a.gbound = h->gbound;
+    a.Kpad = Kpad;
+    h->last_variant = KSVD_OMP_WAVE16;
+    return KSVD_OK;
 }
 
 // OMP::BatchOMP from D and X with the configured variant (compact output).
@@ -196,6 +225,7 @@ int omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float
                       "fast scratch"));
             h->fast = p;
         }
+        h->last_variant = KSVD_OMP_MFMA;
         return hip_check(h,
                          ksvd::launch_batch_omp_fast(X, D, G, c.n, c.K, c.N, c.T, eps2, idx, val, cnt,
                                                      (float)c.pivot_fp64, h->fast, h->fast_grid, h->counter,
@@ -208,6 +238,7 @@ int omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float
     a.idx = idx;
     a.val = val;
     a.cnt = cnt;
+    TRY(prepare_wave(h, G, a));
     if (eps2) {
         TRY(alloc(h, &h->xnorm2, c.N, "xnorm2"));
         hipLaunchKernelGGL(sumsq_cols_kernel, dim3((unsigned)((c.N + 3) / 4)), dim3(256), 0, h->stream, X, c.n, c.N,
@@ -312,6 +343,7 @@ int ksvd_batch_omp(ksvd_handle* h, const float* DtX, const float* G, float* sele
     ksvd::OmpArgs a = omp_args(h, DtX, G);
     a.sel = selected_atoms;
     a.c = c;
+    TRY(prepare_wave(h, G, a));
     return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp");
 }
 
@@ -323,6 +355,7 @@ int ksvd_batch_omp_compact(ksvd_handle* h, const float* DtX, const float* G, int
     a.idx = idx;
     a.val = val;
     a.cnt = cnt;
+    TRY(prepare_wave(h, G, a));
     return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp_compact");
 }
 
@@ -360,11 +393,19 @@ int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float*
     return omp_from_signals(h, D, X, G, eps2, idx, val, cnt);
 }
 
-int ksvd_omp_fallbacks(ksvd_handle* h, int32_t* count) {
+int ksvd_omp_stats(ksvd_handle* h, int64_t* candidates, int64_t* fallbacks) {
     if (!h) return KSVD_ERR_NULL;
-    if (!count) return fail(h, KSVD_ERR_NULL, "omp_fallbacks: null pointer");
-    HIPTRY(h, hipMemcpyAsync(count, h->counter + 1, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream), "download");
-    return hip_check(h, hipStreamSynchronize(h->stream), "sync");
+    unsigned long long st[2] = {0ull, 0ull};
+    int32_t fb = 0;
+    if (h->last_variant == KSVD_OMP_MFMA) {
+        HIPTRY(h, hipMemcpyAsync(&fb, h->counter + 1, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream), "download");
+    } else if (h->omp_stats) {
+        HIPTRY(h, hipMemcpyAsync(st, h->omp_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream), "download");
+    }
+    HIPTRY(h, hipStreamSynchronize(h->stream), "sync");
+    if (candidates) *candidates = (int64_t)st[0];
+    if (fallbacks) *fallbacks = h->last_variant == KSVD_OMP_MFMA ? (int64_t)fb : (int64_t)st[1];
+    return KSVD_OK;
 }
 
 int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const float* xnorm2, const float* eps2,
@@ -379,6 +420,7 @@ int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const f
     a.xnorm2 = xnorm2;
     a.eps2 = eps2;
     a.work_counter = h->counter;
+    TRY(prepare_wave(h, G, a));
     return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp_eps");
 }
 

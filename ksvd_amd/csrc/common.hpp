@@ -109,8 +109,15 @@ struct OmpArgs {
     const float* eps2;
     int32_t* work_counter;  // persistent work-stealing counter (error mode)
     float pivot_fp64;
+    const _Float16* Gh;     // compressed-G form: fp16 G, K columns of Kpad (NULL: FP32 form)
+    const float* gbound;    // compressed-G form: per atom (g_k, d_k)
+    int Kpad;
+    unsigned long long* stats;  // compressed-G form: [0] candidates re-evaluated, [1] FP32 fallbacks
 };
 hipError_t launch_batch_omp(const OmpArgs& a, hipStream_t st);
+int g16_kpad(int K);
+bool g16_supported(int K);
+hipError_t launch_g16_prep(const float* G, int K, _Float16* Gh, float* gbound, hipStream_t st);
 
 // Batch-OMP with the correlation update on MFMA (omp_fast.hip); n <= 256.
 size_t fast_scratch_bytes(int n, int K, int T, int grid);

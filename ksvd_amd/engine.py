@@ -166,10 +166,14 @@ class Engine:
                                                           _p(cnt)))
         return idx, val, cnt
 
+    def omp_stats(self):
+        """(candidates re-evaluated in FP32, fallback steps) of the last Batch-OMP call."""
+        c, f = C.c_int64(), C.c_int64()
+        self._rc(self.lib.ksvd_omp_stats(self.h, C.byref(c), C.byref(f)))
+        return c.value, f.value
+
     def omp_fallbacks(self) -> int:
-        c = C.c_int32()
-        self._rc(self.lib.ksvd_omp_fallbacks(self.h, C.byref(c)))
-        return c.value
+        return self.omp_stats()[1]
 
     def batch_omp_eps(self, DtX, G, xnorm2, eps2, out=None):
         idx, val, cnt = self.empty_compact() if out is None else out

@@ -70,16 +70,18 @@ def test_normcols(o64):
 
 
 # ---- Batch-OMP -----------------------------------------------------------------------------
-def _omp_case(o64, cfg, N, T=None, form="fp32", seed=None):
+def _omp_case(o64, cfg, N, T=None, form="fp32", seed=None, variant="wave"):
     wl, X, D = workload(cfg, N, seed)
     n, K = D.shape[1], D.shape[0]
     T = T or synth.CONFIGS[cfg]["T"]
-    eng = Engine(n, K, N, T, gemm_form=form)
+    eng = Engine(n, K, N, T, gemm_form=form, omp_variant=variant)
     Xg, Dg = cu(X), cu(D)
     G = eng.gram(Dg)
     A0 = eng.correlate(Dg, Xg)
     idx, val, cnt = eng.batch_omp_compact(A0, G)
     torch.cuda.synchronize()
+    cands, falls = eng.omp_stats()
+    say(f"{variant}: candidates re-evaluated {cands}, fallback steps {falls}")
     X64, D64 = X.astype(np.float64), D.astype(np.float64)
     A064 = o64.correlate(D64, X64)
     ref = o64.batch_omp(A064, o64.gram(D64), T, gaps=True)
@@ -105,16 +107,18 @@ def _check_omp(got, ref, A064, tie_rel=1e-4, coef_tol=1e-4, max_tie_frac=0.02):
     return eq / N, rel
 
 
-@pytest.mark.parametrize("cfg,N", [("C0", 4000), ("C1", 2000), ("C2", 1024)])
-def test_batch_omp_parity_fp32(o64, cfg, N):
-    say("batch_omp", cfg, N)
-    got, ref, A064 = _omp_case(o64, cfg, N)
+@pytest.mark.parametrize("variant", ["wave", "wave16"])
+@pytest.mark.parametrize("cfg,N", [("C0", 4000), ("C1", 2000), ("C2", 1024), ("C4", 2000)])
+def test_batch_omp_parity_fp32(o64, cfg, N, variant):
+    say("batch_omp", cfg, N, variant)
+    got, ref, A064 = _omp_case(o64, cfg, N, variant=variant)
     _check_omp(got, ref, A064)
 
 
-def test_batch_omp_bf16x3_parity_c2(o64):
-    say("batch_omp bf16x3 C2 subset")
-    got, ref, A064 = _omp_case(o64, "C2", 2048, form="bf16x3")
+@pytest.mark.parametrize("variant", ["wave", "wave16"])
+def test_batch_omp_bf16x3_parity_c2(o64, variant):
+    say("batch_omp bf16x3 C2 subset", variant)
+    got, ref, A064 = _omp_case(o64, "C2", 2048, form="bf16x3", variant=variant)
     frac, rel = _check_omp(got, ref, A064, tie_rel=1e-3, coef_tol=1e-3, max_tie_frac=0.05)
     say(f"bf16x3 identical-support fraction {frac:.4f}")
 
@@ -179,15 +183,16 @@ def test_batch_omp_mfma_dense_layout(o64):
     eng.close()
 
 
-def test_batch_omp_dense_reference_layout(o64):
+@pytest.mark.parametrize("variant", ["wave", "wave16"])
+def test_batch_omp_dense_reference_layout(o64, variant):
     """SELECTED_ATOMS slot map and c exactly as parOMP writes them, including the
     pivot-break slot that points at a stale c entry."""
-    say("batch_omp dense")
+    say("batch_omp dense", variant)
     E = np.eye(8)
     D = np.stack([E[0], E[0], E[1], E[2]])
     G = (D @ D.T).astype(np.float32)
     A0 = np.array([[3.5, 4.0, 0.1, 0.1], [0.0, 0.0, 2.0, 1.0]], np.float32)
-    eng = Engine(8, 4, 2, 3)
+    eng = Engine(8, 4, 2, 3, omp_variant=variant)
     sel, c = eng.batch_omp(cu(A0), cu(G), c=cu(np.full((2, 3), 7.0, np.float32)))
     ref = o64.batch_omp(A0.astype(np.float64), G.astype(np.float64), 3, dense=True, c_init=np.full((2, 3), 7.0))
     np.testing.assert_array_equal(host(sel), ref["sel"])
@@ -195,7 +200,7 @@ def test_batch_omp_dense_reference_layout(o64):
     eng.close()
     # larger random case: slot map identical where supports agree
     wl, X, Dm = workload("C0", 1000)
-    eng = Engine(64, 256, 1000, 8)
+    eng = Engine(64, 256, 1000, 8, omp_variant=variant)
     G = eng.gram(cu(Dm))
     A0g = eng.correlate(cu(Dm), cu(X))
     sel, c = eng.batch_omp(A0g, G)
@@ -209,11 +214,12 @@ def test_batch_omp_dense_reference_layout(o64):
     eng.close()
 
 
-def test_batch_omp_eps_parity(o64):
-    say("batch_omp_eps C3 subset")
+@pytest.mark.parametrize("variant", ["wave", "wave16"])
+def test_batch_omp_eps_parity(o64, variant):
+    say("batch_omp_eps C3 subset", variant)
     wl, X, D = workload("C3", 3000)
     n, K, N, Tmax = 128, 512, 3000, 48
-    eng = Engine(n, K, N, Tmax)
+    eng = Engine(n, K, N, Tmax, omp_variant=variant)
     Xg, Dg = cu(X), cu(D)
     G = eng.gram(Dg)
     A0 = eng.correlate(Dg, Xg)
