@@ -81,7 +81,9 @@ typedef struct ksvd_config {
     int32_t max_signals;  /* MAX_SIGNALS (5000)                                 */
     int32_t gemm_form;    /* KSVD_FORM_*                                        */
     int32_t omp_variant;  /* KSVD_OMP_*                                         */
-    int32_t reserved0;
+    int32_t use_graph;    /* ksvd_execute_iterations: replay iterations 2.. as */
+                          /* one captured hipGraph (1, default) or launch them  */
+                          /* kernel by kernel (0)                              */
     double mu_thresh;     /* muTHRESH (0.8)                                     */
     double pivot_fp64;    /* Cholesky pivot 1-||w||^2 below this is recomputed  */
                           /* in FP64 (0 disables the fallback)                  */
@@ -89,7 +91,8 @@ typedef struct ksvd_config {
 } ksvd_config;
 
 /* Fills the reference defaults (T=6, 30 iterations, mu=0.8, use_thresh=4,
- * max_signals=5000), FP32 form, pivot_fp64 = 1e-3, perm_seed = 0. */
+ * max_signals=5000), FP32 form, automatic Batch-OMP variant, pivot_fp64 =
+ * 1e-3, perm_seed = 0, use_graph = 1. */
 void ksvd_config_default(ksvd_config* cfg);
 
 /* Handle lifetime.  Replaces the KSVD/CudaAlgorithm constructors

@@ -35,7 +35,7 @@ class Config(C.Structure):
     _fields_ = [
         ("n", C.c_int32), ("K", C.c_int32), ("N", C.c_int32), ("T", C.c_int32),
         ("iterations", C.c_int32), ("use_thresh", C.c_int32), ("max_signals", C.c_int32),
-        ("gemm_form", C.c_int32), ("omp_variant", C.c_int32), ("reserved0", C.c_int32),
+        ("gemm_form", C.c_int32), ("omp_variant", C.c_int32), ("use_graph", C.c_int32),
         ("mu_thresh", C.c_double), ("pivot_fp64", C.c_double), ("perm_seed", C.c_uint64),
     ]
 

@@ -43,6 +43,8 @@ struct ksvd_handle {
     float* gbound = nullptr;   // compressed-G form: per-atom bounds
     unsigned long long* omp_stats = nullptr;  // [0] candidates, [1] fallbacks
     int last_variant = 0;
+    hipStream_t gstream = nullptr;  // private non-blocking stream for graph capture/replay
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 // ||x_s||^2 per signal (one wave per signal).
@@ -266,6 +268,7 @@ void ksvd_config_default(ksvd_config* c) {
     c->mu_thresh = 0.8;
     c->pivot_fp64 = 1e-3;
     c->perm_seed = 0;
+    c->use_graph = 1;
 }
 
 const char* ksvd_version(void) { return "ksvd_amd 0.1 (gfx950)"; }
@@ -294,6 +297,12 @@ int ksvd_create(ksvd_handle** out, const ksvd_config* cfg) {
 int ksvd_destroy(ksvd_handle* h) {
     if (!h) return KSVD_ERR_NULL;
     if (!h->allocs.empty()) hipStreamSynchronize(h->stream);
+    if (h->gstream) {
+        hipStreamSynchronize(h->gstream);
+        hipStreamDestroy(h->gstream);
+    }
+    if (h->ev_in) hipEventDestroy(h->ev_in);
+    if (h->ev_out) hipEventDestroy(h->ev_out);
     for (void* p : h->allocs) hipFree(p);
     delete h;
     return KSVD_OK;
@@ -514,18 +523,71 @@ int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, f
                "perm upload");
     HIPTRY(h, hipStreamSynchronize(h->stream), "perm upload sync");  // `perms` is pageable and local
     HIPTRY(h, ksvd::launch_normcols(D_in, D_out, c.n, c.K, h->stream), "normcols");
-    double* rmse_slot = h->du.rmse_dev;
-    for (int it = 0; it < iterations; ++it) {
+
+    // One iteration with fixed buffers (the iteration's permutation is copied
+    // into du.perm_dev and its RMSE read from du.rmse_dev around it), so the
+    // same launch sequence can be captured once and replayed.
+    auto enqueue = [&]() -> int {
         HIPTRY(h, ksvd::launch_at_times_b(D_out, c.n, c.K, D_out, c.K, h->gram, c.gemm_form, h->stream), "gram");
         TRY(omp_from_signals(h, D_out, X, h->gram, nullptr, idx, val, cnt));
         HIPTRY(h, ksvd::launch_du_init(h->du, h->stream), "du_init");
-        h->du.perm_dev = h->perm_all + (size_t)it * c.K;
         HIPTRY(h, ksvd::launch_du_sweep(h->du, X, D_out, idx, val, cnt, h->stream), "dictionary_update");
-        h->du.rmse_dev = h->rmse_all + it;
-        HIPTRY(h, ksvd::launch_error(h->du, X, D_out, idx, val, cnt, h->errv, h->stream), "error");
+        HIPTRY(h, ksvd::launch_error_from_residual(h->du, h->errv, h->stream), "error");
         HIPTRY(h, ksvd::launch_clear_dict(h->du, X, D_out, idx, val, cnt, h->errv, h->stream), "clear_dict");
+        return KSVD_OK;
+    };
+    auto stage_in = [&](int it) {
+        return hip_check(h, hipMemcpyAsync(h->du.perm_dev, h->perm_all + (size_t)it * c.K, sizeof(uint32_t) * c.K,
+                                           hipMemcpyDeviceToDevice, h->stream),
+                         "perm stage");
+    };
+    auto stage_out = [&](int it) {
+        return hip_check(h, hipMemcpyAsync(h->rmse_all + it, h->du.rmse_dev, sizeof(double), hipMemcpyDeviceToDevice,
+                                           h->stream),
+                         "rmse stage");
+    };
+
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int rc = KSVD_OK;
+    for (int it = 0; it < iterations && rc == KSVD_OK; ++it) {
+        if ((rc = stage_in(it)) != KSVD_OK) break;
+        if (it == 0 || !c.use_graph) {
+            rc = enqueue();  // first iteration eagerly: every buffer gets allocated here
+        } else {
+            if (!exec) {
+                if (!h->gstream) {
+                    HIPTRY(h, hipStreamCreateWithFlags(&h->gstream, hipStreamNonBlocking), "stream");
+                    HIPTRY(h, hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming), "event");
+                    HIPTRY(h, hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming), "event");
+                }
+                hipStream_t user = h->stream;
+                HIPTRY(h, hipStreamBeginCapture(h->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
+                h->stream = h->gstream;
+                const int crc = enqueue();
+                h->stream = user;
+                const hipError_t ec = hipStreamEndCapture(h->gstream, &graph);
+                if (crc != KSVD_OK) { rc = crc; break; }
+                if (ec != hipSuccess) { rc = hip_check(h, ec, "end capture"); break; }
+                if ((rc = hip_check(h, hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate")) !=
+                    KSVD_OK)
+                    break;
+            }
+            // replay on the private stream, ordered after the caller's stream
+            if ((rc = hip_check(h, hipEventRecord(h->ev_in, h->stream), "record")) != KSVD_OK) break;
+            if ((rc = hip_check(h, hipStreamWaitEvent(h->gstream, h->ev_in, 0), "wait")) != KSVD_OK) break;
+            if ((rc = hip_check(h, hipGraphLaunch(exec, h->gstream), "graph launch")) != KSVD_OK) break;
+            if ((rc = hip_check(h, hipEventRecord(h->ev_out, h->gstream), "record")) != KSVD_OK) break;
+            if ((rc = hip_check(h, hipStreamWaitEvent(h->stream, h->ev_out, 0), "wait")) != KSVD_OK) break;
+        }
+        if (rc == KSVD_OK) rc = stage_out(it);
     }
-    h->du.rmse_dev = rmse_slot;
+    if (exec) {
+        hipStreamSynchronize(h->gstream);
+        hipGraphExecDestroy(exec);
+    }
+    if (graph) hipGraphDestroy(graph);
+    if (rc != KSVD_OK) return rc;
     if (rmse_host && iterations > 0) {
         HIPTRY(h, hipMemcpyAsync(rmse_host, h->rmse_all, sizeof(double) * iterations, hipMemcpyDeviceToHost, h->stream),
                "rmse download");

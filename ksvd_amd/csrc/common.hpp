@@ -162,6 +162,7 @@ hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* i
                            const int32_t* cnt, hipStream_t st);
 hipError_t launch_error(DuWork& w, const float* X, const float* D, const int32_t* idx, const float* val,
                         const int32_t* cnt, float* err, hipStream_t st);
+hipError_t launch_error_from_residual(DuWork& w, float* err, hipStream_t st);
 hipError_t launch_clear_dict(DuWork& w, const float* X, float* D, const int32_t* idx, const float* val,
                              const int32_t* cnt, float* err, hipStream_t st);
 hipError_t launch_dense_to_compact(const float* sel, const float* c, int K, int N, int T, int32_t* idx,

@@ -17,6 +17,8 @@
 //   5. gamma_new = (a^T x - u^T Gamma(:,s) + u_p gamma)/||a||, d_p = a/||a||
 // All data-dependent branching is device-side (early exits on N_p), so the
 // host loop never synchronises and the sequence is graph-capturable.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace ksvd {
@@ -429,6 +431,30 @@ __global__ __launch_bounds__(256) void error_kernel(const float* __restrict__ X,
     if (threadIdx.x == 0) part[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
 }
 
+// err_s = ||r_s||^2 from the maintained residuals R = X - D Gamma (after a
+// dictionary-update sweep R holds exactly the reconstruction error the
+// reference forms with a dense D * Gamma GEMM, S/KSVD.cu:235-262).
+__global__ __launch_bounds__(256) void error_from_residual_kernel(const float* __restrict__ R,
+                                                                  const int32_t* __restrict__ bitmap, int n, int N,
+                                                                  float* __restrict__ err, double* __restrict__ part) {
+    __shared__ double wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double acc = 0.0;
+    for (long s = (long)blockIdx.x * 4 + w; s < N; s += (long)gridDim.x * 4) {
+        float e = 0.f;
+        for (int t = lane; t < n; t += 64) {
+            const float r = R[s * n + t];
+            e += r * r;
+        }
+        e = wave_sum(e);
+        if (lane == 0) err[s] = e * (float)bitmap[s];
+        acc += (double)e;
+    }
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
 __global__ __launch_bounds__(1024) void error_finish_kernel(const double* __restrict__ part, int nparts, double denom,
                                                             double* __restrict__ rmse) {
     __shared__ double wred[16];
@@ -439,16 +465,25 @@ __global__ __launch_bounds__(1024) void error_finish_kernel(const double* __rest
 }
 
 // ---- clearDict: S/KSVD.cu:138-226 ------------------------------------------
-// usecount_k = #{s : gamma_ks^2 > 1e-7}  (use_count_kernel, S/KSVD.cu:403-415)
-__global__ void usecount_kernel(const int32_t* __restrict__ idx, const float* __restrict__ val,
-                                const int32_t* __restrict__ cnt, int N, int T, int32_t* __restrict__ usecount) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= N) return;
-    const int c = cnt[s];
-    for (int j = 0; j < c; ++j) {
-        const float v = val[s * T + j];
-        if (v * v > 1e-7f) atomicAdd(&usecount[idx[s * T + j]], 1);
+// usecount_k = #{s : gamma_ks^2 > 1e-7}  (use_count_kernel, S/KSVD.cu:403-415):
+// per-block LDS histogram over a grid-stride range of signals, then one
+// global add per atom and block (integer counts: order independent).
+__global__ __launch_bounds__(256) void usecount_kernel(const int32_t* __restrict__ idx, const float* __restrict__ val,
+                                                       const int32_t* __restrict__ cnt, int N, int T, int K,
+                                                       int32_t* __restrict__ usecount) {
+    extern __shared__ int32_t hist[];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < N; s += (long)gridDim.x * blockDim.x) {
+        const int c = cnt[s];
+        for (int j = 0; j < c; ++j) {
+            const float v = val[s * T + j];
+            if (v * v > 1e-7f) atomicAdd(&hist[idx[s * T + j]], 1);
+        }
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x)
+        if (hist[k]) atomicAdd(&usecount[k], hist[k]);
 }
 
 // Sequential sweep over atoms in one workgroup.  G holds D^T D for the D at
@@ -637,10 +672,19 @@ hipError_t launch_error(DuWork& w, const float* X, const float* D, const int32_t
     return hipGetLastError();
 }
 
+hipError_t launch_error_from_residual(DuWork& w, float* err, hipStream_t st) {
+    hipLaunchKernelGGL(error_from_residual_kernel, dim3(ERR_BLOCKS), dim3(256), 0, st, w.R, w.bitmap, w.n, w.N, err,
+                       w.rmse_part);
+    hipLaunchKernelGGL(error_finish_kernel, dim3(1), dim3(1024), 0, st, w.rmse_part, ERR_BLOCKS,
+                       (double)w.n * (double)w.N, w.rmse_dev);
+    return hipGetLastError();
+}
+
 hipError_t launch_clear_dict(DuWork& w, const float* X, float* D, const int32_t* idx, const float* val,
                              const int32_t* cnt, float* err, hipStream_t st) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(w.K, 256)), dim3(256), 0, st, w.usecount, (long)w.K, 0);
-    hipLaunchKernelGGL(usecount_kernel, dim3(blocks_for(w.N, 256)), dim3(256), 0, st, idx, val, cnt, w.N, w.T,
+    const unsigned ub = (unsigned)std::min<long>(blocks_for(w.N, 256), 512);
+    hipLaunchKernelGGL(usecount_kernel, dim3(ub), dim3(256), (size_t)w.K * 4, st, idx, val, cnt, w.N, w.T, w.K,
                        w.usecount);
     hipError_t e = launch_at_times_b(D, w.n, w.K, D, w.K, w.Gtmp, 0, st);
     if (e != hipSuccess) return e;

@@ -44,11 +44,12 @@ class Engine:
     """One engine handle (ksvd_create) bound to the current torch stream."""
 
     def __init__(self, n, K, N, T, *, iterations=30, gemm_form="fp32", omp_variant="auto", mu_thresh=0.8,
-                 use_thresh=4, max_signals=5000, pivot_fp64=1e-3, perm_seed=0):
+                 use_thresh=4, max_signals=5000, pivot_fp64=1e-3, perm_seed=0, use_graph=True):
         self.lib = _lib.load()
         self.cfg = _lib.default_config(n=n, K=K, N=N, T=T, iterations=iterations, gemm_form=gemm_form,
                                        omp_variant=omp_variant, mu_thresh=mu_thresh, use_thresh=use_thresh,
-                                       max_signals=max_signals, pivot_fp64=pivot_fp64, perm_seed=perm_seed)
+                                       max_signals=max_signals, pivot_fp64=pivot_fp64, perm_seed=perm_seed,
+                                       use_graph=int(bool(use_graph)))
         self.n, self.K, self.N, self.T = n, K, N, T
         h = C.c_void_p()
         rc = self.lib.ksvd_create(C.byref(h), C.byref(self.cfg))

@@ -108,7 +108,7 @@ def _check_omp(got, ref, A064, tie_rel=1e-4, coef_tol=1e-4, max_tie_frac=0.02):
 
 
 @pytest.mark.parametrize("variant", ["wave", "wave16"])
-@pytest.mark.parametrize("cfg,N", [("C0", 4000), ("C1", 2000), ("C2", 1024), ("C4", 2000)])
+@pytest.mark.parametrize("cfg,N", [("C0", 4000), ("C1", 2000), ("C2", 4096), ("C4", 2000)])
 def test_batch_omp_parity_fp32(o64, cfg, N, variant):
     say("batch_omp", cfg, N, variant)
     got, ref, A064 = _omp_case(o64, cfg, N, variant=variant)
@@ -334,26 +334,72 @@ def test_error_and_clear_dict_parity(o64):
     eng.close()
 
 
-def test_execute_iterations_parity(o64):
-    say("execute_iterations C0")
+def _perturbed_init(D0, rel=0.05, seed=3):
+    """D_init plus 5% Gaussian noise per atom: no signal is an exact one-atom fit."""
+    rng = np.random.default_rng(seed)
+    scale = rel * np.linalg.norm(D0, axis=1, keepdims=True) / np.sqrt(D0.shape[1])
+    return (D0 + scale * rng.standard_normal(D0.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_execute_iterations_parity(o64, use_graph):
+    """Four full iterations against the FP64 oracle.  The initial dictionary is
+    perturbed so that no signal is fitted exactly by one atom: with exact fits
+    (D_init drawn from X) the FP64 oracle stops at alpha^2 < 1e-14 while any
+    FP32 implementation adds atoms with ~1e-7 coefficients, which the
+    dictionary update then turns into real updates -- the oracle's own FP32
+    and FP64 builds differ by 2.3e-4 in rmse[0] on that input (checked below)."""
+    say("execute_iterations C0 perturbed init, graph", use_graph)
+    wl = synth.make("C0", N=3000)
+    X = wl.X.numpy()
+    D0 = _perturbed_init(wl.D_init.numpy())
+    eng = Engine(64, 256, 3000, 8, perm_seed=11, use_graph=use_graph)
+    D, idx, val, cnt, rmse = eng.ExecuteIterations(cu(X), cu(D0), 4)
+    ref = o64.ksvd(X.astype(np.float64), D0.astype(np.float64), 8, 4, seed=11)
+    say("rmse gpu", rmse, "oracle", ref["rmse"])
+    np.testing.assert_allclose(rmse, ref["rmse"], rtol=2e-5)
+    assert np.all(np.diff(rmse) < 0)
+    diff = np.abs(host(D) - ref["D"]).max(axis=1)
+    say("4-iteration D column max diff: median", np.median(diff), "max", diff.max())
+    assert np.median(diff) < 1e-4
+    eng.close()
+
+
+def test_execute_iterations_exact_fit_init(o64):
+    """C0 as the reference runs it (D_init = K of the signals): iteration 0 is
+    precision-sensitive (see above), later iterations agree again."""
+    say("execute_iterations C0 exact-fit init")
+    o32 = __import__("oracle").load("f32")
     wl = synth.make("C0", N=3000)
     X, D0 = wl.X.numpy(), wl.D_init.numpy()
     eng = Engine(64, 256, 3000, 8, perm_seed=11)
     D, idx, val, cnt, rmse = eng.ExecuteIterations(cu(X), cu(D0), 4)
     ref = o64.ksvd(X.astype(np.float64), D0.astype(np.float64), 8, 4, seed=11)
-    say("rmse gpu", rmse, "oracle", ref["rmse"])
-    np.testing.assert_allclose(rmse[0], ref["rmse"][0], rtol=1e-4)
-    np.testing.assert_allclose(rmse, ref["rmse"], rtol=2e-3)
-    assert np.all(np.diff(rmse) < 0)
+    ref32 = o32.ksvd(X, D0, 8, 4, seed=11)
+    spread = abs(ref32["rmse"][0] / ref["rmse"][0] - 1)
+    say("rmse gpu", rmse, "oracle f64", ref["rmse"], "oracle f32", ref32["rmse"], "f32/f64 spread", spread)
+    assert spread > 1e-5  # the input is precision-sensitive, as documented
+    np.testing.assert_allclose(rmse[0], ref["rmse"][0], rtol=1e-3)
+    np.testing.assert_allclose(rmse[1:], ref["rmse"][1:], rtol=1e-4)
     eng.close()
-    # one iteration: dictionaries agree closely
-    eng = Engine(64, 256, 3000, 8, perm_seed=11)
-    D, idx, val, cnt, rmse = eng.ExecuteIterations(cu(X), cu(D0), 1)
-    ref = o64.ksvd(X.astype(np.float64), D0.astype(np.float64), 8, 1, seed=11)
-    diff = np.abs(host(D) - ref["D"]).max(axis=1)
-    say("1-iteration D column max diff: median", np.median(diff), "max", diff.max())
-    assert np.median(diff) < 1e-4
-    eng.close()
+
+
+def test_execute_iterations_graph_matches_eager():
+    """The captured-graph replay of iterations >= 2 is bit-identical to eager
+    launches (the whole iteration is deterministic)."""
+    say("execute_iterations graph vs eager")
+    wl = synth.make("C1", N=4000)
+    X, D0 = cu(wl.X.numpy()), cu(wl.D_init.numpy())
+    outs = []
+    for g in (False, True):
+        eng = Engine(240, 512, 4000, 16, perm_seed=7, use_graph=g)
+        D, idx, val, cnt, rmse = eng.ExecuteIterations(X, D0, 5)
+        torch.cuda.synchronize()
+        outs.append((host(D), host(idx), host(val), host(cnt), np.asarray(rmse)))
+        eng.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    say("rmse", outs[0][4])
 
 
 def test_reference_fixture_structure_gpu():
