@@ -4,10 +4,10 @@ Workload (fits one GPU): the precision-form configuration, n=256 signal
 length, K=1024 atoms, N=1,000,000 signals, T=32 atoms per signal, Gaussian
 unit-norm atoms with 25 dB SNR, synthetic and seeded, resident in HBM before
 timing starts.  A step is one Batch-OMP pass over all N signals with the
-headline form: G = D^T D and DtX = D^T X on split-BF16x3 MFMA, then the
-compressed-G wave Batch-OMP kernel (fp16 copy of G streamed for the
-correlation update, exact FP32 selection).  The plain FP32 form (FP32 GEMMs,
-then the FP32-G wave kernel) is timed beside it.
+headline form: G = D^T D and DtX = D^T X on FP32 MFMA, then the register-basis
+Batch-OMP kernel (one workgroup per signal holding Q = G(:,I) L^-T in VGPRs,
+FP32 throughout).  The wave form that streams G every step (FP32 GEMMs, then
+the FP32-G wave kernel) is timed beside it.
 
     python bench.py [--steps 10] [--warmup 2] [--gpus 1]
 
@@ -63,8 +63,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="bf16x3", choices=["fp32", "bf16", "bf16x3"], help="Gram GEMM form")
-    ap.add_argument("--omp", default="wave16", choices=["wave16", "wave", "mfma"], help="headline Batch-OMP kernel")
+    ap.add_argument("--form", default="fp32", choices=["fp32", "bf16", "bf16x3"], help="Gram GEMM form")
+    ap.add_argument("--omp", default="qform", choices=["qform", "wave16", "wave", "mfma"],
+                    help="headline Batch-OMP kernel")
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--ksvd-iters", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain FP32 form (0: skip)")
@@ -137,7 +138,7 @@ def main():
     log(f"step {ms_per_step:.2f} ms (gemms {np.mean(gram_ms):.3f} ms, batch-omp {omp_avg:.2f} ms), "
         f"{value / 1e6:.2f} M signals/s, mean count {c.mean():.2f}, candidates {cands}, fallbacks {fallbacks}")
 
-    # plain FP32 form: FP32 GEMMs (G, DtX) + wave-per-signal Batch-OMP
+    # G-streaming wave form: FP32 GEMMs (G, DtX) + wave-per-signal Batch-OMP
     plain = None
     if args.plain_steps > 0:
         eng_p = Engine(n, K, N, T, gemm_form="fp32", omp_variant="wave")
@@ -184,8 +185,8 @@ def main():
     f_sig = np.array([omp_flops_per_signal(K, t) for t in range(T + 1)])
     flops = float(np.sum(f_sig[c]))
     achieved_tf = flops / (omp_avg / 1000.0) / 1e12
-    kname = {"wave16": "omp_wave16_kernel<16,false>", "wave": "omp_wave_kernel<16,false>",
-             "mfma": "omp_fast_kernel<16>"}[args.omp]
+    kname = {"qform": "omp_qform_kernel<4,32,4,false>", "wave16": "omp_wave16_kernel<16,false>",
+             "wave": "omp_wave_kernel<16,false>", "mfma": "omp_fast_kernel<16>"}[args.omp]
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved_tf, 3),

@@ -54,16 +54,19 @@ enum {
 
 /* Batch-OMP kernel variant. */
 enum {
-    KSVD_OMP_AUTO = 0,   /* compressed-G wave form when K % 4 == 0 and            */
-                         /* 256 <= K <= 1024, else the FP32 wave form             */
+    KSVD_OMP_AUTO = 0,   /* KSVD_OMP_QFORM, except T > 32 with K > 1024 (its basis */
+                         /* would not fit 8 waves' VGPRs): KSVD_OMP_WAVE          */
     KSVD_OMP_WAVE = 1,   /* one wave64 per signal: alpha = alpha0 - G(:,I) gamma   */
                          /* from FP32 columns of G in VGPRs, L in LDS             */
     KSVD_OMP_MFMA = 2,   /* 64-signal tiles: alpha = D^T r on fp16 MFMA with a     */
                          /* rigorous error bound, exact FP32 re-evaluation of the  */
                          /* candidates; entry points taking D and X, n <= 256     */
-    KSVD_OMP_WAVE16 = 3  /* wave form streaming an fp16 copy of G, rigorous error  */
+    KSVD_OMP_WAVE16 = 3, /* wave form streaming an fp16 copy of G, rigorous error  */
                          /* bound per atom, exact FP32 re-evaluation of the        */
                          /* candidates within it (selection = FP32 arg-max)        */
+    KSVD_OMP_QFORM = 4   /* one workgroup per signal holding Q = G(:,I) L^-T in    */
+                         /* VGPRs: alpha -= q_i y_i, one G column per step, no     */
+                         /* per-step triangular solves; FP32 throughout            */
 };
 
 /*
@@ -159,7 +162,8 @@ int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float*
 
 /* Diagnostics of the last Batch-OMP call (synchronises): candidates
  * re-evaluated in FP32 and steps that fell back to the full FP32 update
- * (compressed-G form), or fallbacks of the MFMA form.  NULL outputs allowed. */
+ * (compressed-G form), fallbacks of the MFMA form, or pivots recomputed in
+ * FP64 (register-basis form, candidates 0).  NULL outputs allowed. */
 int ksvd_omp_stats(ksvd_handle* h, int64_t* candidates, int64_t* fallbacks);
 
 /* ---- dictionary update / error / clear ------------------------------------ */

@@ -12,8 +12,8 @@ LIB_PATH = os.path.join(HERE, "libksvd_amd.so")
 
 FORM_FP32, FORM_BF16, FORM_BF16X3 = 0, 1, 2
 FORMS = {"fp32": FORM_FP32, "bf16": FORM_BF16, "bf16x3": FORM_BF16X3}
-OMP_AUTO, OMP_WAVE, OMP_MFMA, OMP_WAVE16 = 0, 1, 2, 3
-OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "mfma": OMP_MFMA, "wave16": OMP_WAVE16}
+OMP_AUTO, OMP_WAVE, OMP_MFMA, OMP_WAVE16, OMP_QFORM = 0, 1, 2, 3, 4
+OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "mfma": OMP_MFMA, "wave16": OMP_WAVE16, "qform": OMP_QFORM}
 
 KSVD_OK = 0
 STATUS = {0: "KSVD_OK", -1: "KSVD_ERR_INVALID", -2: "KSVD_ERR_LIMIT", -3: "KSVD_ERR_HIP", -4: "KSVD_ERR_ALLOC",

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libksvd_amd.so")
-SOURCES = ["gemm.hip", "omp.hip", "omp_fast.hip", "dict_update.hip", "adjoint.hip", "capi.cpp"]
+SOURCES = ["gemm.hip", "omp.hip", "omp_qform.hip", "omp_fast.hip", "dict_update.hip", "adjoint.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KSVD_ARCH", "gfx950")

@@ -109,7 +109,9 @@ int check_cfg(ksvd_handle* h, const ksvd_config* c) {
     if (c->n > 4096) return fail(h, KSVD_ERR_LIMIT, "n=%d exceeds 4096", c->n);
     if ((long)c->N * c->T > 0x7fffffffL) return fail(h, KSVD_ERR_LIMIT, "N*T exceeds 2^31");
     if (c->gemm_form < 0 || c->gemm_form > 2) return fail(h, KSVD_ERR_INVALID, "gemm_form=%d", c->gemm_form);
-    if (c->omp_variant < 0 || c->omp_variant > 3) return fail(h, KSVD_ERR_INVALID, "omp_variant=%d", c->omp_variant);
+    if (c->omp_variant < 0 || c->omp_variant > 4) return fail(h, KSVD_ERR_INVALID, "omp_variant=%d", c->omp_variant);
+    if (c->omp_variant == KSVD_OMP_QFORM && ksvd::qform_shape(c->K, c->T).kpl == 0)
+        return fail(h, KSVD_ERR_LIMIT, "register-basis Batch-OMP needs T <= 32 when K > 1024 (K=%d T=%d)", c->K, c->T);
     if (c->mu_thresh < 0) return fail(h, KSVD_ERR_INVALID, "mu_thresh must be >= 0");
     if (c->max_signals < 1) return fail(h, KSVD_ERR_INVALID, "max_signals must be >= 1");
     return KSVD_OK;
@@ -186,10 +188,13 @@ ksvd::OmpArgs omp_args(ksvd_handle* h, const float* DtX, const float* G) {
 
 bool use_fast(const ksvd_handle* h) { return h->cfg.omp_variant == KSVD_OMP_MFMA && h->cfg.n <= 256; }
 
-bool use_g16(const ksvd_handle* h) {
+bool use_qform(const ksvd_handle* h) {
     const int v = h->cfg.omp_variant;
-    if (v == KSVD_OMP_WAVE16) return ksvd::g16_supported(h->cfg.K);
-    return v == KSVD_OMP_AUTO && ksvd::g16_supported(h->cfg.K) && h->cfg.K >= 256;
+    return (v == KSVD_OMP_QFORM || v == KSVD_OMP_AUTO) && ksvd::qform_shape(h->cfg.K, h->cfg.T).kpl != 0;
+}
+
+bool use_g16(const ksvd_handle* h) {
+    return h->cfg.omp_variant == KSVD_OMP_WAVE16 && ksvd::g16_supported(h->cfg.K);
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p, int n) {
@@ -202,6 +207,10 @@ int prepare_wave(ksvd_handle* h, const float* G, ksvd::OmpArgs& a) {
     TRY(alloc(h, &h->omp_stats, 2, "omp stats"));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, h->stream, h->omp_stats, 2);
     a.stats = h->omp_stats;
+    if (use_qform(h)) {
+        h->last_variant = KSVD_OMP_QFORM;
+        return KSVD_OK;
+    }
     h->last_variant = KSVD_OMP_WAVE;
     if (!use_g16(h)) return KSVD_OK;
     const int Kpad = ksvd::g16_kpad(c.K);
@@ -213,6 +222,12 @@ int prepare_wave(ksvd_handle* h, const float* G, ksvd::OmpArgs& a) {
     a.Kpad = Kpad;
     h->last_variant = KSVD_OMP_WAVE16;
     return KSVD_OK;
+}
+
+// Launches the wave-family kernel that prepare_wave() chose.
+hipError_t launch_wave_family(const ksvd_handle* h, const ksvd::OmpArgs& a) {
+    if (h->last_variant == KSVD_OMP_QFORM) return ksvd::launch_batch_omp_qform(a, h->stream);
+    return ksvd::launch_batch_omp(a, h->stream);
 }
 
 // OMP::BatchOMP from D and X with the configured variant (compact output).
@@ -249,7 +264,7 @@ int omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float
         a.eps2 = eps2;
         a.work_counter = h->counter;
     }
-    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp (wave)");
+    return hip_check(h, launch_wave_family(h, a), "batch_omp (wave)");
 }
 
 }  // namespace
@@ -353,7 +368,7 @@ int ksvd_batch_omp(ksvd_handle* h, const float* DtX, const float* G, float* sele
     a.sel = selected_atoms;
     a.c = c;
     TRY(prepare_wave(h, G, a));
-    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp");
+    return hip_check(h, launch_wave_family(h, a), "batch_omp");
 }
 
 int ksvd_batch_omp_compact(ksvd_handle* h, const float* DtX, const float* G, int32_t* idx, float* val,
@@ -365,7 +380,7 @@ int ksvd_batch_omp_compact(ksvd_handle* h, const float* DtX, const float* G, int
     a.val = val;
     a.cnt = cnt;
     TRY(prepare_wave(h, G, a));
-    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp_compact");
+    return hip_check(h, launch_wave_family(h, a), "batch_omp_compact");
 }
 
 int ksvd_batch_omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G, int32_t* idx,
@@ -430,7 +445,7 @@ int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const f
     a.eps2 = eps2;
     a.work_counter = h->counter;
     TRY(prepare_wave(h, G, a));
-    return hip_check(h, ksvd::launch_batch_omp(a, h->stream), "batch_omp_eps");
+    return hip_check(h, launch_wave_family(h, a), "batch_omp_eps");
 }
 
 int ksvd_du_init(ksvd_handle* h) {

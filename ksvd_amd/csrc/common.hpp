@@ -50,6 +50,30 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 
+// DPP row permutation of a float (all 64 lanes active; every source lane valid
+// for the controls used here).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Wave max without LDS traffic: quad swaps (quad_perm [1,0,3,2], [2,3,0,1]),
+// then row_half_mirror and row_mirror make every 16-lane row uniform; the four
+// row maxima are combined from v_readlane.  Call from uniform control flow.
+__device__ __forceinline__ float wave_max_dpp(float v) {
+    v = fmaxf(v, dpp_f<0xB1>(v));
+    v = fmaxf(v, dpp_f<0x4E>(v));
+    v = fmaxf(v, dpp_f<0x141>(v));
+    v = fmaxf(v, dpp_f<0x140>(v));
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+}
+
+
+
 __device__ __forceinline__ float readlane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -118,6 +142,15 @@ hipError_t launch_batch_omp(const OmpArgs& a, hipStream_t st);
 int g16_kpad(int K);
 bool g16_supported(int K);
 hipError_t launch_g16_prep(const float* G, int K, _Float16* Gh, float* gbound, hipStream_t st);
+
+// Batch-OMP with the Gram-domain basis Q = G(:,I) L^-T in registers
+// (omp_qform.hip): one workgroup of `waves` waves per signal, `kpl` atoms per
+// lane, Q sized for T <= tt.  kpl == 0: (K, T) outside the supported range.
+struct QformShape {
+    int kpl, tt, waves;
+};
+QformShape qform_shape(int K, int T);
+hipError_t launch_batch_omp_qform(const OmpArgs& a, hipStream_t st);
 
 // Batch-OMP with the correlation update on MFMA (omp_fast.hip); n <= 256.
 size_t fast_scratch_bytes(int n, int K, int T, int grid);

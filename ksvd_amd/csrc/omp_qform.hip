@@ -1,0 +1,352 @@
+// omp_qform.hip -- Batch-OMP with the Gram-domain basis held in registers.
+//
+// Same selections and results as parOMP (S/OMP.cu:254-405), computed through
+// a different but algebraically identical recurrence.  With the Cholesky
+// factor L L^T = G_II of the support and y = L^-1 alpha0_I, the reference
+// forms gamma = L^-T y and alpha = alpha0 - G(:,I) gamma every step (a K x i
+// product read from G).  Writing Q = G(:,I) L^-T (K x i), column j of Q is
+// fixed once atom j is in the support, and
+//   alpha^(i)   = alpha^(i-1) - q_i y_i
+//   w           = L^-1 G(I, k_i) = (q_0(k_i), ..., q_{i-1}(k_i))
+//   l_ii        = sqrt(1 - ||w||^2)
+//   q_i         = (G(:, k_i) - sum_j q_j w_j) / l_ii
+//   y_i         = alpha^(i-1)(k_i) / l_ii
+// so a step needs one column of G, the i columns of Q it already holds, and
+// no triangular solve; gamma = L^-T y is solved once at the end.
+//
+// Layout: one workgroup of W waves per signal.  Wave v, lane l holds atoms
+// k = (v*64 + l)*KPL + m, m < KPL, contiguous per lane (vector loads of the
+// DtX row and of G columns), with alpha[m] and Q[j][m] (j < TT) in VGPRs:
+// TT*KPL <= 128 registers, the per-signal working set T*K*4 bytes spread over
+// the workgroup's register files instead of being re-streamed from L2.
+// Per step, one barrier: each wave's arg-max (DPP max + ballot) and its
+// winning lane's row of Q go to LDS (double-buffered by step parity) ->
+// barrier -> every wave reduces the W candidates (same data, same order:
+// identical decisions, so every wave leaves the loop at the same step), loads
+// its slice of G(:, k_i), reads w from the winner's row and forms q_i.
+//
+// The pivot d = 1 - ||w||^2 is recomputed in FP64 from G(I, k_i) and L when
+// it falls below pivot_fp64, as in the wave kernels.
+#include "omp_common.hpp"
+
+namespace ksvd {
+
+namespace {
+
+// out[m] = p[kb + m] for kb + m < K, else 0.  `vec`: 16-byte (or KPL*4-byte)
+// aligned vector loads are legal for this row (K a multiple of min(KPL, 4)).
+template <int KPL>
+__device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, int K, bool vec, float* out) {
+    if (vec && kb + KPL <= K) {
+        if constexpr (KPL == 1) {
+            out[0] = p[kb];
+        } else if constexpr (KPL == 2) {
+            const float2 u = *reinterpret_cast<const float2*>(p + kb);
+            out[0] = u.x; out[1] = u.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < KPL / 4; ++q) {
+                const float4 u = *reinterpret_cast<const float4*>(p + kb + 4 * q);
+                out[4 * q] = u.x; out[4 * q + 1] = u.y; out[4 * q + 2] = u.z; out[4 * q + 3] = u.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < KPL; ++m) out[m] = (kb + m < K) ? p[kb + m] : 0.f;
+    }
+}
+
+// Occupancy asked of the compiler for the largest bases (Q = 128 VGPRs):
+// Q, three KPL-wide arrays (alpha, the G column, Q w) and ~26 others must
+// fit 512/waves registers.  Smaller bases are left to the compiler.
+constexpr int qform_waves_per_eu(int kpl, int tt) {
+    return tt * kpl < 128 ? 1 : 512 / (tt * kpl + 3 * kpl + 26);
+}
+
+template <int KPL, int TT, int W, bool EPS>
+__global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform_kernel(OmpArgs a) {
+    constexpr int LD = TT + 1;
+    __shared__ float Ls[TT * LD];                  // L (row r, col c) at Ls[c*LD + r]
+    // per step parity: each wave's candidate (alpha^2, atom, signed alpha) and
+    // the candidate's row of Q, i.e. w if it wins (zero beyond i)
+    __shared__ float cand_v[2][W], cand_a[2][W];
+    __shared__ int cand_k[2][W];
+    __shared__ __align__(16) float wrow[2][W][TT];
+    extern __shared__ unsigned char slot[];        // K bytes, dense output only
+    const int lane = threadIdx.x & 63;
+    const int v = threadIdx.x >> 6;
+    const int K = a.K, T = a.T;
+    const long s = blockIdx.x;
+    const int kb = (v * 64 + lane) * KPL;
+    const float* __restrict__ G = a.G;
+    const bool vec = (K % (KPL < 4 ? KPL : 4)) == 0;
+
+    float al[KPL];
+    {
+        const float* __restrict__ p = a.DtX + s * (long)K;
+        if (vec && kb + KPL <= K) {
+            if constexpr (KPL == 1) {
+                al[0] = __builtin_nontemporal_load(p + kb);
+            } else if constexpr (KPL == 2) {
+                al[0] = __builtin_nontemporal_load(p + kb);
+                al[1] = __builtin_nontemporal_load(p + kb + 1);
+            } else {
+#pragma unroll
+                for (int q = 0; q < KPL / 4; ++q) {
+                    const f32x4 u = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p + kb + 4 * q));
+                    al[4 * q] = u[0]; al[4 * q + 1] = u[1]; al[4 * q + 2] = u[2]; al[4 * q + 3] = u[3];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < KPL; ++m) al[m] = (kb + m < K) ? p[kb + m] : 0.f;
+        }
+    }
+    float Q[TT][KPL];
+#pragma unroll
+    for (int j = 0; j < TT; ++j)
+#pragma unroll
+        for (int m = 0; m < KPL; ++m) Q[j][m] = 0.f;
+    for (int j = threadIdx.x; j < 2 * W * TT; j += 64 * W) (&wrow[0][0][0])[j] = 0.f;
+    Support sp;
+    double xn2 = 0.0, e2 = 0.0, ysum = 0.0;
+    if constexpr (EPS) {
+        xn2 = (double)a.xnorm2[s];
+        e2 = (double)a.eps2[s];
+    }
+    unsigned n_redo = 0;
+    int brk = -1;  // atom marked in SELECTED_ATOMS by a pivot break (S/OMP.cu:313,331-333)
+    __syncthreads();
+
+    int i = 0;
+    while (i < T) {
+        // error mode: ||x - D_I gamma_I||^2 = ||x||^2 - ||y||^2
+        if constexpr (EPS) {
+            if (xn2 - ysum <= e2) break;
+        }
+        const int b = i & 1;
+        // this wave's arg-max of alpha^2: largest value, lowest atom on ties
+        // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges)
+        float bv = -1.f;
+        int bm = 0;
+#pragma unroll
+        for (int m = 0; m < KPL; ++m) {
+            const float c2 = al[m] * al[m];
+            if (kb + m < K && c2 > bv) { bv = c2; bm = m; }
+        }
+        const float M = wave_max_dpp(bv);
+        const int wl = __builtin_ctzll(__ballot(bv == M));
+        if (lane == wl) {
+            float av = al[0];
+#pragma unroll
+            for (int m = 1; m < KPL; ++m)
+                if (m == bm) av = al[m];
+            cand_v[b][v] = M;
+            cand_k[b][v] = (bv >= 0.f) ? kb + bm : 0x7fffffff;
+            cand_a[b][v] = av;
+#pragma unroll
+            for (int m = 0; m < KPL; ++m) {
+                if (m == bm) {
+#pragma unroll
+                    for (int j = 0; j < TT; ++j)
+                        if (j < i) wrow[b][v][j] = Q[j][m];
+                }
+            }
+        }
+        __syncthreads();
+
+        // the block's arg-max over the W wave winners: strict '>' in wave order
+        // keeps the lowest atom among equal values
+        float best = cand_v[b][0];
+        int pos = cand_k[b][0], vo = 0;
+#pragma unroll
+        for (int u = 1; u < W; ++u) {
+            const float c = cand_v[b][u];
+            if (c > best) { best = c; pos = cand_k[b][u]; vo = u; }
+        }
+        pos = uniform(pos);
+        vo = uniform(vo);
+        best = uniform(best);
+        // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
+        // pos outside [0, K) only when every alpha is NaN
+        if ((unsigned)pos >= (unsigned)K || (double)best < 1e-14) break;
+        if (__ballot(lane < i && sp.ind == pos) != 0ull) break;
+        float g[KPL];
+        load_slice<KPL>(G + (long)pos * K, kb, K, vec, g);
+        const float apos = cand_a[b][vo];
+        // acc = Q w and ||w||^2
+        float acc[KPL];
+#pragma unroll
+        for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
+        float ss = 0.f;
+#pragma unroll
+        for (int j4 = 0; j4 < TT; j4 += 4) {
+            if (j4 < i) {
+                const float4 w4 = *reinterpret_cast<const float4*>(&wrow[b][vo][j4]);
+                ss = fmaf(w4.x, w4.x, ss);
+                ss = fmaf(w4.y, w4.y, ss);
+                ss = fmaf(w4.z, w4.z, ss);
+                ss = fmaf(w4.w, w4.w, ss);
+#pragma unroll
+                for (int m = 0; m < KPL; ++m) {
+                    acc[m] = fmaf(Q[j4][m], w4.x, acc[m]);
+                    acc[m] = fmaf(Q[j4 + 1][m], w4.y, acc[m]);
+                    acc[m] = fmaf(Q[j4 + 2][m], w4.z, acc[m]);
+                    acc[m] = fmaf(Q[j4 + 3][m], w4.w, acc[m]);
+                }
+            }
+        }
+        double dd = (double)(1.f - ss);
+        float wv = (lane < i) ? wrow[b][vo][lane & (TT - 1)] : 0.f;  // w_lane, row i of L
+        if (i > 0 && dd < (double)a.pivot_fp64) {
+            // ill-conditioned: w = L^-1 G(I, pos) and d again in FP64
+            // (backsubt_indexed_append_L / sumSquareValues, S/OMP.cu:169-182, 221-228)
+            n_redo++;
+            double b64 = (lane < i) ? (double)G[(long)pos * K + sp.ind] : 0.0, w64 = 0.0;
+            for (int j = 0; j < i; ++j) {
+                const double wj = readlane(b64, j) / (double)readlane(sp.diag, j);
+                if (lane == j) w64 = wj;
+                if (lane > j && lane < i) b64 = b64 - (double)Ls[j * LD + lane] * wj;
+            }
+            dd = uniform(1.0 - wave_sum((lane < i) ? w64 * w64 : 0.0));
+            wv = (float)w64;
+#pragma unroll
+            for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
+#pragma unroll
+            for (int j = 0; j < TT; ++j) {
+                if (j < i) {
+                    const float wj = readlane(wv, j);
+#pragma unroll
+                    for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j][m], wj, acc[m]);
+                }
+            }
+        }
+        if (dd <= 1e-14) {  // S/OMP.cu:331-333
+            brk = pos;
+            break;
+        }
+        const float l = (float)sqrt(dd);
+        const float rl = 1.f / l;
+        const float y = apos * rl;
+#pragma unroll
+        for (int m = 0; m < KPL; ++m) {
+            const float q = (g[m] - acc[m]) * rl;
+            al[m] = al[m] - q * y;
+            acc[m] = q;
+        }
+#pragma unroll
+        for (int j = 0; j < TT; ++j)
+            if (j == i) {
+#pragma unroll
+                for (int m = 0; m < KPL; ++m) Q[j][m] = acc[m];
+            }
+        if (lane == i) { sp.ind = pos; sp.y = y; sp.diag = l; }
+        if (v == 0) {
+            if (lane < i) Ls[lane * LD + i] = wv;
+            if (lane == i) Ls[i * LD + i] = l;
+        }
+        if constexpr (EPS) ysum += (double)y * (double)y;
+        i++;
+    }
+
+    if (v == 0) {
+        wave_lds_fence();
+        // gamma = L^-T y, lane-parallel backward sweep (backsubst_Lt, S/OMP.cu:126-138)
+        float rhs = sp.y;
+        for (int r = i - 1; r >= 0; --r) {
+            const float gr = readlane(rhs, r) / readlane(sp.diag, r);
+            if (lane == r) sp.gam = gr;
+            if (lane < r) rhs = rhs - Ls[lane * LD + r] * gr;
+        }
+        if (a.idx && lane < T) {
+            a.idx[s * T + lane] = (lane < i) ? sp.ind : -1;
+            a.val[s * T + lane] = (lane < i) ? sp.gam : 0.f;
+        }
+        if (a.cnt && lane == 0) a.cnt[s] = i;
+        if (a.c && lane < i) a.c[s * T + lane] = sp.gam;
+        if (a.stats && lane == 0 && n_redo) atomicAdd(a.stats + 1, (unsigned long long)n_redo);
+    }
+    if (a.sel) {
+        // SELECTED_ATOMS column: slot+1 of each support atom, and the atom a
+        // pivot break marked before stopping
+        for (int k = threadIdx.x; k < K; k += 64 * W) slot[k] = 0;
+        __syncthreads();
+        if (v == 0) {
+            if (lane < i) slot[sp.ind] = (unsigned char)(lane + 1);
+            if (lane == 0 && brk >= 0) slot[brk] = (unsigned char)(i + 1);
+        }
+        __syncthreads();
+        float* selp = a.sel + s * (long)K;
+        for (int k = threadIdx.x; k < K; k += 64 * W) selp[k] = (float)slot[k];
+    }
+}
+
+template <int KPL, int TT, int W>
+hipError_t launch_q(const OmpArgs& a, hipStream_t st) {
+    const size_t lds = a.sel ? (size_t)((a.K + 15) & ~15) : 0;
+    if (a.eps2)
+        hipLaunchKernelGGL((omp_qform_kernel<KPL, TT, W, true>), dim3((unsigned)a.N), dim3(64 * W), lds, st, a);
+    else
+        hipLaunchKernelGGL((omp_qform_kernel<KPL, TT, W, false>), dim3((unsigned)a.N), dim3(64 * W), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int KPL, int TT>
+hipError_t launch_qw(const OmpArgs& a, int W, hipStream_t st) {
+    switch (W) {
+        case 1: return launch_q<KPL, TT, 1>(a, st);
+        case 2: return launch_q<KPL, TT, 2>(a, st);
+        case 4: return launch_q<KPL, TT, 4>(a, st);
+        case 8: return launch_q<KPL, TT, 8>(a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+QformShape qform_shape(int K, int T) {
+    QformShape sh{0, 0, 0};
+    if (K < 1 || T < 1 || T > 64) return sh;
+    const int TT = T <= 8 ? 8 : T <= 16 ? 16 : T <= 32 ? 32 : 64;
+    const int kmax = (128 / TT) < 8 ? 128 / TT : 8;
+    const int need = (K + 63) / 64;
+    int kpl = 1;
+    while (kpl < need && kpl < kmax) kpl *= 2;
+    const int waves = (K + 64 * kpl - 1) / (64 * kpl);
+    int W = 1;
+    while (W < waves) W *= 2;
+    if (W > 8) return sh;  // 16 waves would cap a wave at 128 VGPRs: Q alone needs them
+    sh.kpl = kpl;
+    sh.tt = TT;
+    sh.waves = W;
+    return sh;
+}
+
+hipError_t launch_batch_omp_qform(const OmpArgs& a, hipStream_t st) {
+    if (a.N <= 0) return hipSuccess;
+    const QformShape sh = qform_shape(a.K, a.T);
+    if (!sh.kpl) return hipErrorInvalidValue;
+    const int W = sh.waves;
+    switch (sh.tt) {
+        case 8:
+            if (sh.kpl == 1) return launch_q<1, 8, 1>(a, st);
+            if (sh.kpl == 2) return launch_q<2, 8, 1>(a, st);
+            if (sh.kpl == 4) return launch_q<4, 8, 1>(a, st);
+            return launch_qw<8, 8>(a, W, st);
+        case 16:
+            if (sh.kpl == 1) return launch_q<1, 16, 1>(a, st);
+            if (sh.kpl == 2) return launch_q<2, 16, 1>(a, st);
+            if (sh.kpl == 4) return launch_q<4, 16, 1>(a, st);
+            return launch_qw<8, 16>(a, W, st);
+        case 32:
+            if (sh.kpl == 1) return launch_q<1, 32, 1>(a, st);
+            if (sh.kpl == 2) return launch_q<2, 32, 1>(a, st);
+            return launch_qw<4, 32>(a, W, st);
+        case 64:
+            if (sh.kpl == 1) return launch_q<1, 64, 1>(a, st);
+            return launch_qw<2, 64>(a, W, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ksvd

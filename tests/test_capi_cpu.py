@@ -67,3 +67,18 @@ def test_null_and_invalid_arguments_are_reported():
     h = C.c_void_p()
     assert lib.ksvd_create(C.byref(h), C.byref(cfg)) == -2
     lib.ksvd_destroy(h)
+
+
+def test_register_basis_variant_limits():
+    """KSVD_OMP_QFORM holds T x K FP32 values per signal in at most 8 waves'
+    VGPRs: T > 32 with K > 1024 is rejected before any device call."""
+    lib = _lib.load()
+    h = C.c_void_p()
+    cfg = _lib.default_config(n=64, K=2048, N=10, T=40, omp_variant="qform")
+    assert lib.ksvd_create(C.byref(h), C.byref(cfg)) == -2
+    assert b"K=2048 T=40" in lib.ksvd_last_error(h)
+    lib.ksvd_destroy(h)
+    cfg = _lib.default_config(n=64, K=99, N=10, T=5, omp_variant=5)
+    h = C.c_void_p()
+    assert lib.ksvd_create(C.byref(h), C.byref(cfg)) == -1
+    lib.ksvd_destroy(h)

@@ -107,7 +107,7 @@ def _check_omp(got, ref, A064, tie_rel=1e-4, coef_tol=1e-4, max_tie_frac=0.02):
     return eq / N, rel
 
 
-@pytest.mark.parametrize("variant", ["wave", "wave16"])
+@pytest.mark.parametrize("variant", ["wave", "wave16", "qform"])
 @pytest.mark.parametrize("cfg,N", [("C0", 4000), ("C1", 2000), ("C2", 4096), ("C4", 2000)])
 def test_batch_omp_parity_fp32(o64, cfg, N, variant):
     say("batch_omp", cfg, N, variant)
@@ -115,7 +115,7 @@ def test_batch_omp_parity_fp32(o64, cfg, N, variant):
     _check_omp(got, ref, A064)
 
 
-@pytest.mark.parametrize("variant", ["wave", "wave16"])
+@pytest.mark.parametrize("variant", ["wave", "wave16", "qform"])
 def test_batch_omp_bf16x3_parity_c2(o64, variant):
     say("batch_omp bf16x3 C2 subset", variant)
     got, ref, A064 = _omp_case(o64, "C2", 2048, form="bf16x3", variant=variant)
@@ -183,7 +183,7 @@ def test_batch_omp_mfma_dense_layout(o64):
     eng.close()
 
 
-@pytest.mark.parametrize("variant", ["wave", "wave16"])
+@pytest.mark.parametrize("variant", ["wave", "wave16", "qform"])
 def test_batch_omp_dense_reference_layout(o64, variant):
     """SELECTED_ATOMS slot map and c exactly as parOMP writes them, including the
     pivot-break slot that points at a stale c entry."""
@@ -214,7 +214,7 @@ def test_batch_omp_dense_reference_layout(o64, variant):
     eng.close()
 
 
-@pytest.mark.parametrize("variant", ["wave", "wave16"])
+@pytest.mark.parametrize("variant", ["wave", "wave16", "qform"])
 def test_batch_omp_eps_parity(o64, variant):
     say("batch_omp_eps C3 subset", variant)
     wl, X, D = workload("C3", 3000)
