@@ -57,6 +57,25 @@ __device__ __forceinline__ float dpp_f(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
+// DPP row permutation of an int (same controls as dpp_f).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+// Signed-int wave max (DPP within rows, v_readlane across the four rows);
+// on the bit patterns of floats that are >= 0 or negative sentinels this is
+// the float max.  Call from uniform control flow.
+__device__ __forceinline__ int wave_max_i32_dpp(int v) {
+    v = max(v, dpp_i<0xB1>(v));
+    v = max(v, dpp_i<0x4E>(v));
+    v = max(v, dpp_i<0x141>(v));
+    v = max(v, dpp_i<0x140>(v));
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return max(max(r0, r1), max(r2, r3));
+}
+
 // Wave max without LDS traffic: quad swaps (quad_perm [1,0,3,2], [2,3,0,1]),
 // then row_half_mirror and row_mirror make every 16-lane row uniform; the four
 // row maxima are combined from v_readlane.  Call from uniform control flow.

@@ -66,13 +66,16 @@ constexpr int qform_waves_per_eu(int kpl, int tt) {
 template <int KPL, int TT, int W, bool EPS>
 __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform_kernel(OmpArgs a) {
     constexpr int LD = TT + 1;
-    __shared__ float Ls[TT * LD];                  // L (row r, col c) at Ls[c*LD + r]
+    constexpr int UG = 4;        // steps per unrolled group (TT is a multiple)
+    constexpr int NG = TT / UG;  // groups of UG support slots
+    __shared__ float Ls[TT * LD];  // L (row r, col c) at Ls[c*LD + r]
     // per step parity: each wave's candidate (alpha^2, atom, signed alpha) and
-    // the candidate's row of Q, i.e. w if it wins (zero beyond i)
-    __shared__ float cand_v[2][W], cand_a[2][W];
-    __shared__ int cand_k[2][W];
+    // the candidate's row of Q (zero beyond the step), which is w if it wins
+    __shared__ int cand_v[2][W], cand_k[2][W];
+    __shared__ float cand_a[2][W];
     __shared__ __align__(16) float wrow[2][W][TT];
-    extern __shared__ unsigned char slot[];        // K bytes, dense output only
+    __shared__ __align__(16) float wfix[W][TT];  // per wave: w recomputed in FP64 (zero beyond the step)
+    extern __shared__ unsigned char slot[];      // K bytes, dense output only
     const int lane = threadIdx.x & 63;
     const int v = threadIdx.x >> 6;
     const int K = a.K, T = a.T;
@@ -81,8 +84,12 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
     const float* __restrict__ G = a.G;
     const bool vec = (K % (KPL < 4 ? KPL : 4)) == 0;
 
+    // alpha of this lane's atoms (0 for padding atoms k >= K: they sit above
+    // every real atom, so with the lowest-index tie rule they never win
+    // against one)
     float al[KPL];
     {
+        // the DtX row is read once: non-temporal, so it does not evict G
         const float* __restrict__ p = a.DtX + s * (long)K;
         if (vec && kb + KPL <= K) {
             if constexpr (KPL == 1) {
@@ -108,6 +115,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
 #pragma unroll
         for (int m = 0; m < KPL; ++m) Q[j][m] = 0.f;
     for (int j = threadIdx.x; j < 2 * W * TT; j += 64 * W) (&wrow[0][0][0])[j] = 0.f;
+    for (int j = threadIdx.x; j < W * TT; j += 64 * W) (&wfix[0][0])[j] = 0.f;
     Support sp;
     double xn2 = 0.0, e2 = 0.0, ysum = 0.0;
     if constexpr (EPS) {
@@ -118,136 +126,171 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
     int brk = -1;  // atom marked in SELECTED_ATOMS by a pivot break (S/OMP.cu:313,331-333)
     __syncthreads();
 
-    int i = 0;
-    while (i < T) {
-        // error mode: ||x - D_I gamma_I||^2 = ||x||^2 - ||y||^2
-        if constexpr (EPS) {
-            if (xn2 - ysum <= e2) break;
-        }
-        const int b = i & 1;
-        // this wave's arg-max of alpha^2: largest value, lowest atom on ties
-        // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges)
-        float bv = -1.f;
-        int bm = 0;
+    // Steps i = UG*c + u: u is a compile-time constant (inner loop unrolled),
+    // so indexing Q by i costs an NG-long chain over c instead of TT.
+    int cnt = 0;
+    bool go = true;
+    for (int c = 0; go && c < NG; ++c) {
 #pragma unroll
-        for (int m = 0; m < KPL; ++m) {
-            const float c2 = al[m] * al[m];
-            if (kb + m < K && c2 > bv) { bv = c2; bm = m; }
-        }
-        const float M = wave_max_dpp(bv);
-        const int wl = __builtin_ctzll(__ballot(bv == M));
-        if (lane == wl) {
-            float av = al[0];
-#pragma unroll
-            for (int m = 1; m < KPL; ++m)
-                if (m == bm) av = al[m];
-            cand_v[b][v] = M;
-            cand_k[b][v] = (bv >= 0.f) ? kb + bm : 0x7fffffff;
-            cand_a[b][v] = av;
+        for (int u = 0; u < UG; ++u) {
+            const int i = UG * c + u;
+            // groups of UG rows of Q holding slots < i
+            const int ng = c + (u > 0 ? 1 : 0);
+            if (i >= T) { go = false; break; }
+            // error mode: ||x - D_I gamma_I||^2 = ||x||^2 - ||y||^2
+            if constexpr (EPS) {
+                if (xn2 - ysum <= e2) { go = false; break; }
+            }
+            const int b = u & 1;  // step parity (UG*c is even; constant after unrolling)
+            // this wave's arg-max of alpha^2: largest value, lowest atom on ties
+            // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges).
+            // alpha^2 >= 0, so its bit pattern orders like a signed int.
+            int bv = -1, bm = 0;
 #pragma unroll
             for (int m = 0; m < KPL; ++m) {
-                if (m == bm) {
-#pragma unroll
-                    for (int j = 0; j < TT; ++j)
-                        if (j < i) wrow[b][v][j] = Q[j][m];
-                }
+                const int c2 = __float_as_int(al[m] * al[m]);
+                if (c2 > bv) { bv = c2; bm = m; }
             }
-        }
-        __syncthreads();
-
-        // the block's arg-max over the W wave winners: strict '>' in wave order
-        // keeps the lowest atom among equal values
-        float best = cand_v[b][0];
-        int pos = cand_k[b][0], vo = 0;
+            const int M = wave_max_i32_dpp(bv);
+            const int wl = __builtin_ctzll(__ballot(bv == M));
+            if (lane == wl) {
+                float av = al[0];
 #pragma unroll
-        for (int u = 1; u < W; ++u) {
-            const float c = cand_v[b][u];
-            if (c > best) { best = c; pos = cand_k[b][u]; vo = u; }
-        }
-        pos = uniform(pos);
-        vo = uniform(vo);
-        best = uniform(best);
-        // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
-        // pos outside [0, K) only when every alpha is NaN
-        if ((unsigned)pos >= (unsigned)K || (double)best < 1e-14) break;
-        if (__ballot(lane < i && sp.ind == pos) != 0ull) break;
-        float g[KPL];
-        load_slice<KPL>(G + (long)pos * K, kb, K, vec, g);
-        const float apos = cand_a[b][vo];
-        // acc = Q w and ||w||^2
-        float acc[KPL];
-#pragma unroll
-        for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
-        float ss = 0.f;
-#pragma unroll
-        for (int j4 = 0; j4 < TT; j4 += 4) {
-            if (j4 < i) {
-                const float4 w4 = *reinterpret_cast<const float4*>(&wrow[b][vo][j4]);
-                ss = fmaf(w4.x, w4.x, ss);
-                ss = fmaf(w4.y, w4.y, ss);
-                ss = fmaf(w4.z, w4.z, ss);
-                ss = fmaf(w4.w, w4.w, ss);
+                for (int m = 1; m < KPL; ++m)
+                    if (m == bm) av = al[m];
+                cand_v[b][v] = M;
+                cand_k[b][v] = kb + bm;
+                cand_a[b][v] = av;
+                // rows j < i of Q for atom kb + bm (groups of UG; rows >= i are zero)
 #pragma unroll
                 for (int m = 0; m < KPL; ++m) {
-                    acc[m] = fmaf(Q[j4][m], w4.x, acc[m]);
-                    acc[m] = fmaf(Q[j4 + 1][m], w4.y, acc[m]);
-                    acc[m] = fmaf(Q[j4 + 2][m], w4.z, acc[m]);
-                    acc[m] = fmaf(Q[j4 + 3][m], w4.w, acc[m]);
+                    if (m == bm) {
+#pragma unroll
+                        for (int g = 0; g < NG; ++g) {
+                            if (g < ng) {
+#pragma unroll
+                                for (int e = 0; e < UG; ++e) wrow[b][v][UG * g + e] = Q[UG * g + e][m];
+                            }
+                        }
+                    }
                 }
             }
-        }
-        double dd = (double)(1.f - ss);
-        float wv = (lane < i) ? wrow[b][vo][lane & (TT - 1)] : 0.f;  // w_lane, row i of L
-        if (i > 0 && dd < (double)a.pivot_fp64) {
-            // ill-conditioned: w = L^-1 G(I, pos) and d again in FP64
-            // (backsubt_indexed_append_L / sumSquareValues, S/OMP.cu:169-182, 221-228)
-            n_redo++;
-            double b64 = (lane < i) ? (double)G[(long)pos * K + sp.ind] : 0.0, w64 = 0.0;
-            for (int j = 0; j < i; ++j) {
-                const double wj = readlane(b64, j) / (double)readlane(sp.diag, j);
-                if (lane == j) w64 = wj;
-                if (lane > j && lane < i) b64 = b64 - (double)Ls[j * LD + lane] * wj;
+            __syncthreads();
+
+            // the block's arg-max over the W wave winners: strict '>' in wave
+            // order keeps the lowest atom among equal values
+            int best = cand_v[b][0];
+            int pos = cand_k[b][0], vo = 0;
+#pragma unroll
+            for (int w2 = 1; w2 < W; ++w2) {
+                const int cv = cand_v[b][w2];
+                if (cv > best) { best = cv; pos = cand_k[b][w2]; vo = w2; }
             }
-            dd = uniform(1.0 - wave_sum((lane < i) ? w64 * w64 : 0.0));
-            wv = (float)w64;
+            pos = uniform(pos);
+            vo = uniform(vo);
+            const float bestf = __int_as_float(uniform(best));
+            // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
+            // pos >= K: only padding atoms left (every real alpha NaN or 0)
+            if ((unsigned)pos >= (unsigned)K || !((double)bestf >= 1e-14)) { go = false; break; }
+            if (__ballot(lane < i && sp.ind == pos) != 0ull) { go = false; break; }
+            float g[KPL];
+            load_slice<KPL>(G + (long)pos * K, kb, K, vec, g);
+            const float apos = cand_a[b][vo];
+            // w_j = q_j(pos): d = 1 - ||w||^2 and acc = Q w in one pass
+            const float* wsrc = wrow[b][vo];
+            float acc[KPL];
 #pragma unroll
             for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
+            float ss = 0.f;
 #pragma unroll
-            for (int j = 0; j < TT; ++j) {
-                if (j < i) {
-                    const float wj = readlane(wv, j);
+            for (int gg = 0; gg < NG; ++gg) {
+                if (gg < ng) {
 #pragma unroll
-                    for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j][m], wj, acc[m]);
+                    for (int h = 0; h < UG / 4; ++h) {
+                        const float4 w4 = *reinterpret_cast<const float4*>(wsrc + UG * gg + 4 * h);
+                        const int j = UG * gg + 4 * h;
+                        ss = fmaf(w4.x, w4.x, ss);
+                        ss = fmaf(w4.y, w4.y, ss);
+                        ss = fmaf(w4.z, w4.z, ss);
+                        ss = fmaf(w4.w, w4.w, ss);
+#pragma unroll
+                        for (int m = 0; m < KPL; ++m) {
+                            acc[m] = fmaf(Q[j][m], w4.x, acc[m]);
+                            acc[m] = fmaf(Q[j + 1][m], w4.y, acc[m]);
+                            acc[m] = fmaf(Q[j + 2][m], w4.z, acc[m]);
+                            acc[m] = fmaf(Q[j + 3][m], w4.w, acc[m]);
+                        }
+                    }
                 }
             }
-        }
-        if (dd <= 1e-14) {  // S/OMP.cu:331-333
-            brk = pos;
-            break;
-        }
-        const float l = (float)sqrt(dd);
-        const float rl = 1.f / l;
-        const float y = apos * rl;
+            double dd = (double)(1.f - ss);
+            float wv = (lane < i) ? wsrc[lane & (TT - 1)] : 0.f;  // w_lane: row i of L
+            if (i > 0 && dd < (double)a.pivot_fp64) {
+                // ill-conditioned: w = L^-1 G(I, pos) and d again in FP64
+                // (backsubt_indexed_append_L / sumSquareValues, S/OMP.cu:169-182, 221-228)
+                n_redo++;
+                double b64 = (lane < i) ? (double)G[(long)pos * K + sp.ind] : 0.0, w64 = 0.0;
+                for (int j = 0; j < i; ++j) {
+                    const double wj = readlane(b64, j) / (double)readlane(sp.diag, j);
+                    if (lane == j) w64 = wj;
+                    if (lane > j && lane < i) b64 = b64 - (double)Ls[j * LD + lane] * wj;
+                }
+                dd = uniform(1.0 - wave_sum((lane < i) ? w64 * w64 : 0.0));
+                wv = (float)w64;
+                if (lane < i) wfix[v][lane] = wv;
+                wave_lds_fence();
 #pragma unroll
-        for (int m = 0; m < KPL; ++m) {
-            const float q = (g[m] - acc[m]) * rl;
-            al[m] = al[m] - q * y;
-            acc[m] = q;
-        }
+                for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
 #pragma unroll
-        for (int j = 0; j < TT; ++j)
-            if (j == i) {
+                for (int gg = 0; gg < NG; ++gg) {
+                    if (gg < ng) {
 #pragma unroll
-                for (int m = 0; m < KPL; ++m) Q[j][m] = acc[m];
+                        for (int h = 0; h < UG / 4; ++h) {
+                            const float4 w4 = *reinterpret_cast<const float4*>(&wfix[v][UG * gg + 4 * h]);
+                            const int j = UG * gg + 4 * h;
+#pragma unroll
+                            for (int m = 0; m < KPL; ++m) {
+                                acc[m] = fmaf(Q[j][m], w4.x, acc[m]);
+                                acc[m] = fmaf(Q[j + 1][m], w4.y, acc[m]);
+                                acc[m] = fmaf(Q[j + 2][m], w4.z, acc[m]);
+                                acc[m] = fmaf(Q[j + 3][m], w4.w, acc[m]);
+                            }
+                        }
+                    }
+                }
             }
-        if (lane == i) { sp.ind = pos; sp.y = y; sp.diag = l; }
-        if (v == 0) {
-            if (lane < i) Ls[lane * LD + i] = wv;
-            if (lane == i) Ls[i * LD + i] = l;
+            if (dd <= 1e-14) {  // S/OMP.cu:331-333
+                brk = pos;
+                go = false;
+                break;
+            }
+            // q_i = (G(:, pos) - Q w) / l_ii,  alpha -= q_i y_i
+            const float l = (float)sqrt(dd);
+            const float rl = 1.f / l;
+            const float y = apos * rl;
+#pragma unroll
+            for (int m = 0; m < KPL; ++m) {
+                const float q = (g[m] - acc[m]) * rl;
+                al[m] = al[m] - q * y;
+                acc[m] = q;
+            }
+            // Q(:, UG*c + u) = q
+#pragma unroll
+            for (int cc = 0; cc < NG; ++cc)
+                if (cc == c) {
+#pragma unroll
+                    for (int m = 0; m < KPL; ++m) Q[UG * cc + u][m] = acc[m];
+                }
+            if (lane == i) { sp.ind = pos; sp.y = y; sp.diag = l; }
+            if (v == 0) {
+                if (lane < i) Ls[lane * LD + i] = wv;
+                if (lane == i) Ls[i * LD + i] = l;
+            }
+            if constexpr (EPS) ysum += (double)y * (double)y;
+            cnt = i + 1;
         }
-        if constexpr (EPS) ysum += (double)y * (double)y;
-        i++;
     }
+    const int i = cnt;
 
     if (v == 0) {
         wave_lds_fence();
