@@ -76,6 +76,22 @@ __device__ __forceinline__ int wave_max_i32_dpp(int v) {
     return max(max(r0, r1), max(r2, r3));
 }
 
+// Wave sum with DPP row operations (quad swaps, half-row and row mirrors:
+// every lane of a row ends with the same row sum, the adds being commutative
+// pairs), then the four row sums in fixed order.  Deterministic; uniform
+// control flow only.
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
+}
+
 // Wave max without LDS traffic: quad swaps (quad_perm [1,0,3,2], [2,3,0,1]),
 // then row_half_mirror and row_mirror make every 16-lane row uniform; the four
 // row maxima are combined from v_readlane.  Call from uniform control flow.
@@ -192,7 +208,7 @@ struct DuWork {
     float* R;            // n x N restricted residuals X - D Gamma
     float* apart;        // DU_BLOCKS x n partial a
     float* bpart;        // DU_BLOCKS x 2 partial arg-max (special case)
-    float* dnew;         // n
+    float* dnew;         // 2 x n: new column, by parity of the sweep position
     float* scal;         // 8 scalars (norm etc.)
     int32_t* tickets;    // 4 last-block-done counters (zero between launches)
     int32_t* unused;     // N

@@ -410,6 +410,288 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
     for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = dnew[t];
 }
 
+// ---- per-atom phases, n <= 1024: batched, latency-hiding forms -------------
+// Same arithmetic as du_phase_a/b_kernel above, organised for memory-level
+// parallelism: each wave loads up to 64 of its list entries (position and
+// gamma) lane-parallel, then streams FQ residual rows at a time (one float4
+// per lane per 256 columns); 8 waves per block.  Phase A's block partials are
+// reduced by the last block in a fixed order (deterministic).  Phase B does not
+// write d_p: the next atom's phase A copies the new column in (its block 0),
+// and du_tail_kernel does it for the last atom of the sweep; nothing reads
+// d_p in between.
+constexpr int DU_WAVES = 8;
+constexpr int FQ = 4;  // residual rows in flight per wave
+
+__device__ __forceinline__ float4 ld4(const float* __restrict__ p, int t, int n, bool vec) {
+    if (vec) return (t < n) ? *reinterpret_cast<const float4*>(p + t) : make_float4(0.f, 0.f, 0.f, 0.f);
+    return make_float4(t < n ? p[t] : 0.f, t + 1 < n ? p[t + 1] : 0.f, t + 2 < n ? p[t + 2] : 0.f,
+                       t + 3 < n ? p[t + 3] : 0.f);
+}
+
+__device__ __forceinline__ void st4(float* __restrict__ p, int t, int n, bool vec, float4 v) {
+    if (vec) {
+        if (t < n) *reinterpret_cast<float4*>(p + t) = v;
+        return;
+    }
+    if (t < n) p[t] = v.x;
+    if (t + 1 < n) p[t + 1] = v.y;
+    if (t + 2 < n) p[t + 2] = v.z;
+    if (t + 3 < n) p[t + 3] = v.w;
+}
+
+__device__ __forceinline__ float4 f4fma(float a, float4 x, float4 y) {
+    return make_float4(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y), fmaf(a, x.z, y.z), fmaf(a, x.w, y.w));
+}
+
+// Copies the new column of the atom updated at sweep position jj (if it was a
+// regular update) from dnew[jj & 1] into D.
+__device__ __forceinline__ void du_commit_column(int jj, const uint32_t* __restrict__ perm,
+                                                 const int32_t* __restrict__ count, float* __restrict__ D,
+                                                 const float* __restrict__ dnew, int n) {
+    const int pp = (int)perm[jj];
+    if (count[pp] > 0)
+        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)pp * n + t] = dnew[(long)(jj & 1) * n + t];
+}
+
+template <int NCH>
+__global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                   const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
+                                                   const int32_t* __restrict__ list_pos,
+                                                   const float* __restrict__ val, const float* __restrict__ X,
+                                                   float* __restrict__ D, const float* __restrict__ R,
+                                                   int32_t* __restrict__ unused, int32_t* __restrict__ usc,
+                                                   int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
+                                                   int N, int T, int M, float* __restrict__ apart,
+                                                   float* __restrict__ bpart, float* __restrict__ dnew,
+                                                   int32_t* __restrict__ ticket) {
+    __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a
+    __shared__ float red[4][256];                // last block: sub-range sums
+    __shared__ float aval[256 * NCH];            // last block: a
+    __shared__ float rv[DU_WAVES];
+    __shared__ int ri[DU_WAVES];
+    __shared__ int flag_sh;
+    __shared__ float wred[16];
+    const int p = (int)perm[jj];
+    const int Np = count[p];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const bool vec = (n & 3) == 0;
+    if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, dnew, n);
+    if (Np == 0) {
+        // unused-atom special case (S/DictionaryUpdate.cu:238-302, SCupdate :406-427)
+        const int counter = *usc;
+        const int lim = min(M, N - counter);
+        float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
+        int bi = 0;
+        for (int b = blockIdx.x * DU_WAVES + w; b < lim; b += G * DU_WAVES) {
+            const long sig = unused[b];
+            float e = 0.f;
+            for (int t = lane; t < n; t += 64) {
+                const float r = R[sig * n + t];
+                e += r * r;
+            }
+            e = wave_sum_dpp(e);
+            if (e > bv) { bv = e; bi = b; }  // b ascending per wave: keeps the lowest on ties
+        }
+        if (lane == 0) { rv[w] = bv; ri[w] = bi; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float v = rv[0];
+            int i = ri[0];
+            for (int q = 1; q < DU_WAVES; ++q)
+                if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
+            bpart[2 * blockIdx.x] = v;
+            bpart[2 * blockIdx.x + 1] = __int_as_float(i);
+        }
+        if (!last_block(ticket, &flag_sh)) return;
+        if (threadIdx.x == 0) {
+            float v = bpart[0];
+            int i = __float_as_int(bpart[1]);
+            for (int q = 1; q < G; ++q) {
+                const float vq = bpart[2 * q];
+                const int iq = __float_as_int(bpart[2 * q + 1]);
+                if (vq > v || (vq == v && iq < i)) { v = vq; i = iq; }
+            }
+            ri[0] = i;
+        }
+        __syncthreads();
+        const int best = ri[0];
+        const long sig = unused[best];
+        float sq = 0.f;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const float x = X[sig * n + t];
+            sq += x * x;
+        }
+        sq = block_sum(sq, wred);
+        const float nrm = sqrtf(sq);
+        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unused[best] = unused[N - counter - 1];
+            *usc = counter + 1;
+            replaced[p] = 1;
+            bitmap[sig] = 0;
+        }
+        return;
+    }
+    // a = sum_{s in I_p} gamma_s (r_s + d_p gamma_s)
+    const int beg = ptr[p];
+    const int gw = blockIdx.x * DU_WAVES + w, GW = G * DU_WAVES;
+    const float* dp = D + (long)p * n;
+    float4 dv[NCH], acc[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        dv[c] = ld4(dp, 256 * c + 4 * lane, n, vec);
+        acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    for (int base = gw; base < Np; base += 64 * GW) {
+        const int mk = base + lane * GW;
+        const int pos = (mk < Np) ? list_pos[beg + mk] : 0;
+        const float gl = (mk < Np) ? val[pos] : 0.f;
+        const int sl = pos / T;
+        const int nk = min(64, (Np - base + GW - 1) / GW);
+        for (int k = 0; k < nk; k += FQ) {
+            float4 r[FQ][NCH];
+            float g[FQ];
+#pragma unroll
+            for (int u = 0; u < FQ; ++u) {
+                const bool ok = k + u < nk;
+                const long s = readlane(sl, ok ? k + u : 0);
+                g[u] = ok ? readlane(gl, k + u) : 0.f;
+#pragma unroll
+                for (int c = 0; c < NCH; ++c)
+                    r[u][c] = ok ? ld4(R + s * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < FQ; ++u)
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) acc[c] = f4fma(g[u], f4fma(g[u], dv[c], r[u][c]), acc[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) part[w][64 * c + lane] = acc[c];
+    __syncthreads();
+    const float* pf = reinterpret_cast<const float*>(&part[0][0]);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        float sacc = 0.f;
+#pragma unroll
+        for (int q = 0; q < DU_WAVES; ++q) sacc += pf[q * 256 * NCH + t];
+        apart[(long)blockIdx.x * n + t] = sacc;
+    }
+    if (!last_block(ticket, &flag_sh)) return;
+    // sum of the G block partials in a fixed order: nsub sub-ranges per column
+    const int nsub = n <= 128 ? 4 : n <= 256 ? 2 : 1;
+    float sq = 0.f;
+    for (int t0 = 0; t0 < n; t0 += 256) {
+        const int c = t0 + (threadIdx.x & 255), sub = threadIdx.x >> 8;  // sub in {0, 1}
+        for (int half = 0; half < 2; ++half) {
+            const int sb = sub + 2 * half;  // sub-range index 0..3
+            if (sb < nsub && c < n) {
+                const int b0 = G * sb / nsub, b1 = G * (sb + 1) / nsub;
+                float sacc = 0.f;
+                for (int b = b0; b < b1; ++b) sacc += apart[(long)b * n + c];
+                red[sb][c - t0] = sacc;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 256 && c < n) {
+            float a = red[0][threadIdx.x];
+            for (int q = 1; q < nsub; ++q) a += red[q][threadIdx.x];
+            aval[c] = a;
+            sq += a * a;
+        }
+        __syncthreads();
+    }
+    sq = block_sum(sq, wred);
+    const float nrm = sqrtf(sq);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) dnew[(long)(jj & 1) * n + t] = aval[t] / nrm;
+}
+
+template <int NCH>
+__global__ __launch_bounds__(512) void du_b_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                   const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
+                                                   const int32_t* __restrict__ list_pos, float* __restrict__ val,
+                                                   const float* __restrict__ D, float* __restrict__ R,
+                                                   const float* __restrict__ dnew, int n, int T) {
+    const int p = (int)perm[jj];
+    const int Np = count[p];
+    if (Np == 0) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool vec = (n & 3) == 0;
+    const int beg = ptr[p];
+    const int gw = blockIdx.x * DU_WAVES + w, GW = gridDim.x * DU_WAVES;
+    const float* dp = D + (long)p * n;
+    const float* dq = dnew + (long)(jj & 1) * n;
+    float4 dv[NCH], dn[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        dv[c] = ld4(dp, 256 * c + 4 * lane, n, vec);
+        dn[c] = ld4(dq, 256 * c + 4 * lane, n, vec);
+    }
+    for (int base = gw; base < Np; base += 64 * GW) {
+        const int mk = base + lane * GW;
+        const int pos = (mk < Np) ? list_pos[beg + mk] : 0;
+        const float gl = (mk < Np) ? val[pos] : 0.f;
+        const int sl = pos / T;
+        const int nk = min(64, (Np - base + GW - 1) / GW);
+        for (int k = 0; k < nk; k += FQ) {
+            float4 e[FQ][NCH];
+            float dot[FQ];
+#pragma unroll
+            for (int u = 0; u < FQ; ++u) {
+                const bool ok = k + u < nk;
+                const long s = readlane(sl, ok ? k + u : 0);
+                const float g = ok ? readlane(gl, k + u) : 0.f;
+                dot[u] = 0.f;
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    // e = r_s + d_p gamma_s
+                    e[u][c] = ok ? f4fma(g, dv[c], ld4(R + s * n, 256 * c + 4 * lane, n, vec))
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < FQ; ++u) {
+#pragma unroll
+                for (int c = 0; c < NCH; ++c) {
+                    dot[u] = fmaf(dn[c].x, e[u][c].x, dot[u]);
+                    dot[u] = fmaf(dn[c].y, e[u][c].y, dot[u]);
+                    dot[u] = fmaf(dn[c].z, e[u][c].z, dot[u]);
+                    dot[u] = fmaf(dn[c].w, e[u][c].w, dot[u]);
+                }
+                dot[u] = wave_sum_dpp(dot[u]);  // gamma_new = d_new^T (r_s + d_p gamma_s)
+            }
+#pragma unroll
+            for (int u = 0; u < FQ; ++u) {
+                if (k + u < nk) {
+                    const long s = readlane(sl, k + u);
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c)
+                        st4(R + s * n, 256 * c + 4 * lane, n, vec, f4fma(-dot[u], dn[c], e[u][c]));
+                    if (lane == 0) val[readlane(pos, k + u)] = dot[u];
+                }
+            }
+        }
+    }
+}
+
+__global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const int32_t* __restrict__ count,
+                               float* __restrict__ D, const float* __restrict__ dnew, int n) {
+    du_commit_column(jj, perm, count, D, dnew, n);
+}
+
+template <int NCH>
+void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hipStream_t st) {
+    for (int jj = 0; jj < w.K; ++jj) {
+        hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
+                           w.list_pos, val, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T, M,
+                           w.apart, w.bpart, w.dnew, w.tickets);
+        hipLaunchKernelGGL((du_b_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
+                           w.list_pos, val, D, w.R, w.dnew, w.n, w.T);
+    }
+    hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, w.dnew, w.n);
+}
+
 // ---- error: S/KSVD.cu:235-262, S/ReductionKernels.cu:257-322 ---------------
 constexpr int ERR_BLOCKS = 2048;
 
@@ -652,6 +934,13 @@ hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* i
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(du_residual_kernel, dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n, w.N, w.T, w.R);
     const int M = w.N < w.max_signals ? w.N : w.max_signals;
+    if (w.n <= 1024) {
+        const int nch = (w.n + 255) / 256;
+        if (nch == 1) launch_du_atoms<1>(w, X, D, val, M, st);
+        else if (nch == 2) launch_du_atoms<2>(w, X, D, val, M, st);
+        else launch_du_atoms<4>(w, X, D, val, M, st);
+        return hipGetLastError();
+    }
     const size_t lds_a = (size_t)4 * w.n * sizeof(float);
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL(du_phase_a_kernel, dim3(DU_BLOCKS), dim3(256), lds_a, st, jj, w.perm_dev, w.count, w.ptr,
