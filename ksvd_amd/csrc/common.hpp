@@ -206,7 +206,7 @@ struct DuWork {
     int32_t* chunk_hist; // K x nchunks
     int32_t* list_pos;   // N*T
     float* R;            // n x N restricted residuals X - D Gamma
-    float* apart;        // DU_BLOCKS x n partial a
+    float* apart;        // DU_BLOCKS x (n rounded up to 4) partial a
     float* bpart;        // DU_BLOCKS x 2 partial arg-max (special case)
     float* dnew;         // 2 x n: new column, by parity of the sweep position
     float* scal;         // 8 scalars (norm etc.)

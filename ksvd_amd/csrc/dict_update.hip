@@ -420,7 +420,8 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
 // and du_tail_kernel does it for the last atom of the sweep; nothing reads
 // d_p in between.
 constexpr int DU_WAVES = 8;
-constexpr int FQ = 4;  // residual rows in flight per wave
+constexpr int FQ = 4;   // residual rows in flight per wave (phase B: rows kept for the update)
+constexpr int FQA = 8;  // residual rows in flight per wave (phase A)
 
 __device__ __forceinline__ float4 ld4(const float* __restrict__ p, int t, int n, bool vec) {
     if (vec) return (t < n) ? *reinterpret_cast<const float4*>(p + t) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -465,8 +466,6 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
                                                    float* __restrict__ bpart, float* __restrict__ dnew,
                                                    int32_t* __restrict__ ticket) {
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a
-    __shared__ float red[4][256];                // last block: sub-range sums
-    __shared__ float aval[256 * NCH];            // last block: a
     __shared__ float rv[DU_WAVES];
     __shared__ int ri[DU_WAVES];
     __shared__ int flag_sh;
@@ -550,11 +549,11 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
         const float gl = (mk < Np) ? val[pos] : 0.f;
         const int sl = pos / T;
         const int nk = min(64, (Np - base + GW - 1) / GW);
-        for (int k = 0; k < nk; k += FQ) {
-            float4 r[FQ][NCH];
-            float g[FQ];
+        for (int k = 0; k < nk; k += FQA) {
+            float4 r[FQA][NCH];
+            float g[FQA];
 #pragma unroll
-            for (int u = 0; u < FQ; ++u) {
+            for (int u = 0; u < FQA; ++u) {
                 const bool ok = k + u < nk;
                 const long s = readlane(sl, ok ? k + u : 0);
                 g[u] = ok ? readlane(gl, k + u) : 0.f;
@@ -563,7 +562,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
                     r[u][c] = ok ? ld4(R + s * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
-            for (int u = 0; u < FQ; ++u)
+            for (int u = 0; u < FQA; ++u)
 #pragma unroll
                 for (int c = 0; c < NCH; ++c) acc[c] = f4fma(g[u], f4fma(g[u], dv[c], r[u][c]), acc[c]);
         }
@@ -571,41 +570,82 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
 #pragma unroll
     for (int c = 0; c < NCH; ++c) part[w][64 * c + lane] = acc[c];
     __syncthreads();
+    // block partial, rows padded to np4 = n rounded up to 4 floats
+    const int np4 = (n + 3) & ~3;
     const float* pf = reinterpret_cast<const float*>(&part[0][0]);
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    for (int t = threadIdx.x; t < np4; t += blockDim.x) {
         float sacc = 0.f;
 #pragma unroll
         for (int q = 0; q < DU_WAVES; ++q) sacc += pf[q * 256 * NCH + t];
-        apart[(long)blockIdx.x * n + t] = sacc;
+        apart[(long)blockIdx.x * np4 + t] = sacc;
     }
-    if (!last_block(ticket, &flag_sh)) return;
-    // sum of the G block partials in a fixed order: nsub sub-ranges per column
-    const int nsub = n <= 128 ? 4 : n <= 256 ? 2 : 1;
-    float sq = 0.f;
-    for (int t0 = 0; t0 < n; t0 += 256) {
-        const int c = t0 + (threadIdx.x & 255), sub = threadIdx.x >> 8;  // sub in {0, 1}
-        for (int half = 0; half < 2; ++half) {
-            const int sb = sub + 2 * half;  // sub-range index 0..3
-            if (sb < nsub && c < n) {
-                const int b0 = G * sb / nsub, b1 = G * (sb + 1) / nsub;
-                float sacc = 0.f;
-                for (int b = b0; b < b1; ++b) sacc += apart[(long)b * n + c];
-                red[sb][c - t0] = sacc;
+}
+
+// Sum of phase A's G block partials in a fixed order (one workgroup):
+// thread = (slice, group of 4 columns); each slice of partials is summed with
+// 8 independent accumulators (8 loads in flight), slices are combined in
+// order; then ||a|| and d_new = a / ||a|| (S/Multiplication.cu:221) into
+// dnew[jj & 1].  A separate launch (instead of a last-block ticket) keeps
+// device-scope fences out of phase A.
+template <int NCH>
+__global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const uint32_t* __restrict__ perm,
+                                                          const int32_t* __restrict__ count,
+                                                          const float* __restrict__ apart, float* __restrict__ dnew,
+                                                          int n, int G) {
+    __shared__ float4 red[2048];       // slice sums, [slice][column group]
+    __shared__ float aval[256 * NCH];  // a
+    __shared__ float wred[16];
+    const int p = (int)perm[jj];
+    if (count[p] == 0) return;
+    const int np4 = (n + 3) & ~3;
+    const int ng4 = np4 / 4;
+    const int nsl = max(1, min(8, (int)blockDim.x / ng4));
+    for (int g0 = 0; g0 < ng4; g0 += (int)blockDim.x / nsl) {
+        const int cg = g0 + (int)threadIdx.x % ((int)blockDim.x / nsl), sl = (int)threadIdx.x / ((int)blockDim.x / nsl);
+        if (sl < nsl && cg < ng4) {
+            const int b0 = G * sl / nsl, b1 = G * (sl + 1) / nsl;
+            float4 s8[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s8[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+            int b = b0;
+            for (; b + 8 <= b1; b += 8) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float4 v4 = *reinterpret_cast<const float4*>(apart + (long)(b + e) * np4 + 4 * cg);
+                    s8[e] = make_float4(s8[e].x + v4.x, s8[e].y + v4.y, s8[e].z + v4.z, s8[e].w + v4.w);
+                }
             }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (b + e < b1) {
+                    const float4 v4 = *reinterpret_cast<const float4*>(apart + (long)(b + e) * np4 + 4 * cg);
+                    s8[e] = make_float4(s8[e].x + v4.x, s8[e].y + v4.y, s8[e].z + v4.z, s8[e].w + v4.w);
+                }
+            }
+            float4 t4 = s8[0];
+#pragma unroll
+            for (int e = 1; e < 8; ++e) t4 = make_float4(t4.x + s8[e].x, t4.y + s8[e].y, t4.z + s8[e].z, t4.w + s8[e].w);
+            red[sl * ((int)blockDim.x / nsl) + (cg - g0)] = t4;
         }
         __syncthreads();
-        if (threadIdx.x < 256 && c < n) {
-            float a = red[0][threadIdx.x];
-            for (int q = 1; q < nsub; ++q) a += red[q][threadIdx.x];
-            aval[c] = a;
-            sq += a * a;
+        const int gcount = (int)blockDim.x / nsl;
+        if ((int)threadIdx.x < gcount && g0 + (int)threadIdx.x < ng4) {
+            float4 t4 = red[threadIdx.x];
+            for (int q = 1; q < nsl; ++q) {
+                const float4 v4 = red[q * gcount + threadIdx.x];
+                t4 = make_float4(t4.x + v4.x, t4.y + v4.y, t4.z + v4.z, t4.w + v4.w);
+            }
+            reinterpret_cast<float4*>(aval)[g0 + threadIdx.x] = t4;
         }
         __syncthreads();
     }
+    float sq = 0.f;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) sq += aval[t] * aval[t];
     sq = block_sum(sq, wred);
     const float nrm = sqrtf(sq);
     for (int t = threadIdx.x; t < n; t += blockDim.x) dnew[(long)(jj & 1) * n + t] = aval[t] / nrm;
 }
+
 
 template <int NCH>
 __global__ __launch_bounds__(512) void du_b_kernel(int jj, const uint32_t* __restrict__ perm,
@@ -686,6 +726,8 @@ void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hip
         hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
                            w.list_pos, val, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T, M,
                            w.apart, w.bpart, w.dnew, w.tickets);
+        hipLaunchKernelGGL((du_a_reduce_kernel<NCH>), dim3(1), dim3(512), 0, st, jj, w.perm_dev, w.count, w.apart,
+                           w.dnew, w.n, DU_BLOCKS);
         hipLaunchKernelGGL((du_b_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
                            w.list_pos, val, D, w.R, w.dnew, w.n, w.T);
     }
