@@ -16,7 +16,7 @@
 //
 // Layout: one workgroup of W waves per signal.  Wave v, lane l holds atoms
 // k = (v*64 + l)*KPL + m, m < KPL, contiguous per lane (vector loads of the
-// DtX row and of G columns), with alpha[m] and Q[j][m] (j < TT) in VGPRs:
+// DtX row and of G columns), with alpha[m] and Q[j][m] (j < TT-1) in VGPRs:
 // TT*KPL <= 128 registers, the per-signal working set T*K*4 bytes spread over
 // the workgroup's register files instead of being re-streamed from L2.
 // Per step, one barrier: each wave's arg-max (DPP max + ballot) and its
@@ -69,6 +69,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
     constexpr int UG = 4;        // steps per unrolled group (TT is a multiple)
     constexpr int NG = TT / UG;  // groups of UG support slots
     __shared__ float Ls[TT * LD];  // L (row r, col c) at Ls[c*LD + r]
+    __shared__ float ys[TT];       // y = L^-1 alpha0_I
     // per step parity: each wave's candidate (alpha^2, atom, signed alpha) and
     // the candidate's row of Q (zero beyond the step), which is w if it wins
     __shared__ int cand_v[2][W], cand_k[2][W];
@@ -109,9 +110,12 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
             for (int m = 0; m < KPL; ++m) al[m] = (kb + m < K) ? p[kb + m] : 0.f;
         }
     }
-    float Q[TT][KPL];
+    // columns 0 .. TT-2 of Q: the column of step TT-1 (necessarily the last
+    // step, T <= TT) is never read
+    constexpr int QT = TT - 1;
+    float Q[QT][KPL];
 #pragma unroll
-    for (int j = 0; j < TT; ++j)
+    for (int j = 0; j < QT; ++j)
 #pragma unroll
         for (int m = 0; m < KPL; ++m) Q[j][m] = 0.f;
     for (int j = threadIdx.x; j < 2 * W * TT; j += 64 * W) (&wrow[0][0][0])[j] = 0.f;
@@ -169,7 +173,8 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
                         for (int g = 0; g < NG; ++g) {
                             if (g < ng) {
 #pragma unroll
-                                for (int e = 0; e < UG; ++e) wrow[b][v][UG * g + e] = Q[UG * g + e][m];
+                                for (int e = 0; e < UG; ++e)
+                                    if (UG * g + e < QT) wrow[b][v][UG * g + e] = Q[UG * g + e][m];
                             }
                         }
                     }
@@ -213,25 +218,27 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
                         ss = fmaf(w4.y, w4.y, ss);
                         ss = fmaf(w4.z, w4.z, ss);
                         ss = fmaf(w4.w, w4.w, ss);
+                        const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-                        for (int m = 0; m < KPL; ++m) {
-                            acc[m] = fmaf(Q[j][m], w4.x, acc[m]);
-                            acc[m] = fmaf(Q[j + 1][m], w4.y, acc[m]);
-                            acc[m] = fmaf(Q[j + 2][m], w4.z, acc[m]);
-                            acc[m] = fmaf(Q[j + 3][m], w4.w, acc[m]);
-                        }
+                        for (int e = 0; e < 4; ++e)
+                            if (j + e < QT) {
+#pragma unroll
+                                for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j + e][m], wj[e], acc[m]);
+                            }
                     }
                 }
             }
-            double dd = (double)(1.f - ss);
-            float wv = (lane < i) ? wsrc[lane & (TT - 1)] : 0.f;  // w_lane: row i of L
-            if (i > 0 && dd < (double)a.pivot_fp64) {
+            const float d = 1.f - ss;
+            double dd = (double)d;
+            const bool redo = i > 0 && dd < (double)a.pivot_fp64;
+            float wv = 0.f;  // w_lane: row i of L (written to Ls by wave 0)
+            if (redo) {
                 // ill-conditioned: w = L^-1 G(I, pos) and d again in FP64
                 // (backsubt_indexed_append_L / sumSquareValues, S/OMP.cu:169-182, 221-228)
                 n_redo++;
                 double b64 = (lane < i) ? (double)G[(long)pos * K + sp.ind] : 0.0, w64 = 0.0;
                 for (int j = 0; j < i; ++j) {
-                    const double wj = readlane(b64, j) / (double)readlane(sp.diag, j);
+                    const double wj = readlane(b64, j) / (double)Ls[j * LD + j];
                     if (lane == j) w64 = wj;
                     if (lane > j && lane < i) b64 = b64 - (double)Ls[j * LD + lane] * wj;
                 }
@@ -248,25 +255,35 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
                         for (int h = 0; h < UG / 4; ++h) {
                             const float4 w4 = *reinterpret_cast<const float4*>(&wfix[v][UG * gg + 4 * h]);
                             const int j = UG * gg + 4 * h;
+                            const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
-                            for (int m = 0; m < KPL; ++m) {
-                                acc[m] = fmaf(Q[j][m], w4.x, acc[m]);
-                                acc[m] = fmaf(Q[j + 1][m], w4.y, acc[m]);
-                                acc[m] = fmaf(Q[j + 2][m], w4.z, acc[m]);
-                                acc[m] = fmaf(Q[j + 3][m], w4.w, acc[m]);
-                            }
+                            for (int e = 0; e < 4; ++e)
+                                if (j + e < QT) {
+#pragma unroll
+                                    for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j + e][m], wj[e], acc[m]);
+                                }
                         }
                     }
                 }
+            } else if (v == 0) {
+                wv = (lane < i) ? wsrc[lane & (TT - 1)] : 0.f;
             }
             if (dd <= 1e-14) {  // S/OMP.cu:331-333
                 brk = pos;
                 go = false;
                 break;
             }
-            // q_i = (G(:, pos) - Q w) / l_ii,  alpha -= q_i y_i
-            const float l = (float)sqrt(dd);
-            const float rl = 1.f / l;
+            // q_i = (G(:, pos) - Q w) / l_ii,  alpha -= q_i y_i.  1/l_ii from
+            // v_rsq_f32 (1 ulp) on the FP32 pivot, from FP64 after a recompute
+            float rl, l;
+            if (redo) {
+                const double sq = sqrt(dd);
+                l = (float)sq;
+                rl = (float)(1.0 / sq);
+            } else {
+                rl = __builtin_amdgcn_rsqf(d);
+                l = d * rl;
+            }
             const float y = apos * rl;
 #pragma unroll
             for (int m = 0; m < KPL; ++m) {
@@ -278,13 +295,15 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
 #pragma unroll
             for (int cc = 0; cc < NG; ++cc)
                 if (cc == c) {
+                    if (UG * cc + u < QT) {
 #pragma unroll
-                    for (int m = 0; m < KPL; ++m) Q[UG * cc + u][m] = acc[m];
+                        for (int m = 0; m < KPL; ++m) Q[UG * cc + u][m] = acc[m];
+                    }
                 }
-            if (lane == i) { sp.ind = pos; sp.y = y; sp.diag = l; }
+            if (lane == i) sp.ind = pos;
             if (v == 0) {
                 if (lane < i) Ls[lane * LD + i] = wv;
-                if (lane == i) Ls[i * LD + i] = l;
+                if (lane == i) { Ls[i * LD + i] = l; ys[i] = y; }
             }
             if constexpr (EPS) ysum += (double)y * (double)y;
             cnt = i + 1;
@@ -295,9 +314,9 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
     if (v == 0) {
         wave_lds_fence();
         // gamma = L^-T y, lane-parallel backward sweep (backsubst_Lt, S/OMP.cu:126-138)
-        float rhs = sp.y;
+        float rhs = (lane < i) ? ys[lane & (TT - 1)] : 0.f;
         for (int r = i - 1; r >= 0; --r) {
-            const float gr = readlane(rhs, r) / readlane(sp.diag, r);
+            const float gr = readlane(rhs, r) / Ls[r * LD + r];
             if (lane == r) sp.gam = gr;
             if (lane < r) rhs = rhs - Ls[lane * LD + r] * gr;
         }
