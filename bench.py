@@ -40,6 +40,25 @@ def omp_flops_per_signal(K, T):
     return K * T * (T + 1) + T * (T + 1) * (2 * T + 1) / 2
 
 
+def profiled_traffic(kernel_src):
+    """HBM bytes per launch of the Batch-OMP kernel from the committed counter
+    summary (profiles/omp_qform_n256_k1024_t32.md), used only when that summary
+    was made from the current kernel source (sha256 recorded there)."""
+    import hashlib
+    import re
+    path = os.path.join(ROOT, "profiles", "omp_qform_n256_k1024_t32.md")
+    try:
+        text = open(path).read()
+        src = open(os.path.join(ROOT, "ksvd_amd", "csrc", kernel_src), "rb").read()
+    except OSError:
+        return None
+    h = re.search(r"kernel source sha256: ([0-9a-f]{64})", text)
+    t = re.search(r"traffic_bytes_per_launch: ([0-9.e+]+)", text)
+    if not h or not t or h.group(1) != hashlib.sha256(src).hexdigest():
+        return None
+    return float(t.group(1))
+
+
 def cpu_baseline(X_sample, D, T, threads):
     """FP32 port (oracle/liboracle_f32.so) of the reference Batch-OMP, OpenMP
     over signals: G, DtX and Batch-OMP on the sample."""
@@ -71,7 +90,18 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain FP32 form (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run one device copy of GIB GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+        x = torch.ones(int(args.copy_probe * (1 << 28)), device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (DESIGN.md describes how it would shard)",
               file=sys.stderr)
@@ -193,12 +223,14 @@ def main():
         "peak": FP32_PEAK_TF,
         "unit": "TFLOP/s",
         "frac": round(achieved_tf / FP32_PEAK_TF, 4),
-        "traffic": None,
+        "traffic": profiled_traffic("omp_qform.hip") if args.omp == "qform" else None,
         "kernel": kname,
         "kernel_ms": round(omp_avg, 3),
         "work_per_launch": f"F_omp = K T(T+1) + T(T+1)(2T+1)/2 = {omp_flops_per_signal(K, T):.0f} FP32 FLOP per "
                            f"signal at T={T}, summed over the {N} signals' support sizes",
-        "note": "FP32 vector work (no MFMA-eligible form), priced at MI355X's 157.3 TFLOP/s FP32 dense peak",
+        "note": "FP32 vector FMAs (per-signal basis recurrence, no shared operand for MFMA), priced at MI355X's "
+                "157.3 TFLOP/s FP32 peak; non-packed FP32 VALU issue tops out at half of that; traffic from the "
+                "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (null if the kernel source changed since)",
     }
 
     cpu = None
