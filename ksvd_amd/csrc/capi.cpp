@@ -136,10 +136,11 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.ptr, c.K + 1, "ptr"));
     TRY(alloc(h, &w.chunk_hist, (size_t)c.K * w.nchunks, "chunk_hist"));
     TRY(alloc(h, &w.list_pos, (size_t)c.N * c.T, "list_pos"));
+    TRY(alloc(h, &w.list_g, (size_t)c.N * c.T, "list_g"));
     TRY(alloc(h, &w.R, (size_t)c.n * c.N, "R"));
     TRY(alloc(h, &w.apart, (size_t)256 * ((c.n + 3) & ~3), "apart"));
     TRY(alloc(h, &w.bpart, (size_t)2 * 256, "bpart"));
-    TRY(alloc(h, &w.dnew, (size_t)2 * c.n, "dnew"));
+    TRY(alloc(h, &w.dnew, (size_t)2 * c.n + 32, "dnew"));
     TRY(alloc(h, &w.scal, 8, "scal"));
     TRY(alloc(h, &w.tickets, 4, "tickets"));
     hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, h->stream, w.tickets, 4);

@@ -205,10 +205,11 @@ struct DuWork {
     int32_t* ptr;        // K + 1
     int32_t* chunk_hist; // K x nchunks
     int32_t* list_pos;   // N*T
+    float* list_g;       // N*T: gamma of each list entry at list-build time
     float* R;            // n x N restricted residuals X - D Gamma
     float* apart;        // DU_BLOCKS x (n rounded up to 4) partial a
     float* bpart;        // DU_BLOCKS x 2 partial arg-max (special case)
-    float* dnew;         // 2 x n: new column, by parity of the sweep position
+    float* dnew;         // 2 x n + 2 x 16: a and its partial sums of squares, by sweep-position parity
     float* scal;         // 8 scalars (norm etc.)
     int32_t* tickets;    // 4 last-block-done counters (zero between launches)
     int32_t* unused;     // N
@@ -225,7 +226,8 @@ struct DuWork {
     int rmse_parts;
 };
 hipError_t launch_du_init(DuWork& w, hipStream_t st);
-hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* cnt, hipStream_t st);
+hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* cnt, hipStream_t st,
+                                 const float* val = nullptr);
 hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* idx, float* val,
                            const int32_t* cnt, hipStream_t st);
 hipError_t launch_error(DuWork& w, const float* X, const float* D, const int32_t* idx, const float* val,

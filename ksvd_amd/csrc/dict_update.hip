@@ -122,7 +122,8 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
                                                      int N, int T, int K, int CH,
                                                      const int32_t* __restrict__ chunk_off,
                                                      const int32_t* __restrict__ ptr,
-                                                     int32_t* __restrict__ list_pos) {
+                                                     int32_t* __restrict__ list_pos,
+                                                     const float* __restrict__ val, float* __restrict__ list_g) {
     extern __shared__ int32_t cur[];
     const int lane = threadIdx.x;
     const long c = blockIdx.x;
@@ -143,6 +144,7 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
             const int p = cur[kk];
             cur[kk] = p + 1;
             list_pos[p] = (int32_t)(s * T + lane);
+            if (list_g) list_g[p] = val[s * T + lane];
         }
         wave_lds_fence();
     }
@@ -422,6 +424,7 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
 constexpr int DU_WAVES = 8;
 constexpr int FQ = 4;   // residual rows in flight per wave (phase B: rows kept for the update)
 constexpr int FQA = 8;  // residual rows in flight per wave (phase A)
+constexpr int DU_RB = 16;  // workgroups of the phase A reduction
 
 __device__ __forceinline__ float4 ld4(const float* __restrict__ p, int t, int n, bool vec) {
     if (vec) return (t < n) ? *reinterpret_cast<const float4*>(p + t) : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -444,27 +447,40 @@ __device__ __forceinline__ float4 f4fma(float a, float4 x, float4 y) {
     return make_float4(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y), fmaf(a, x.z, y.z), fmaf(a, x.w, y.w));
 }
 
-// Copies the new column of the atom updated at sweep position jj (if it was a
-// regular update) from dnew[jj & 1] into D.
+// ||a|| of the atom at sweep position jj from the DU_RB per-block partial sums
+// of squares (fixed order: every consumer computes the same value).
+__device__ __forceinline__ float du_anorm(int jj, const float* __restrict__ anorm) {
+    const float* q = anorm + (jj & 1) * DU_RB;
+    float s = 0.f;
+#pragma unroll
+    for (int r = 0; r < DU_RB; ++r) s += q[r];
+    return sqrtf(s);
+}
+
+// Copies the new column d_new = a / ||a|| of the atom updated at sweep
+// position jj (if it was a regular update) into D.
 __device__ __forceinline__ void du_commit_column(int jj, const uint32_t* __restrict__ perm,
                                                  const int32_t* __restrict__ count, float* __restrict__ D,
-                                                 const float* __restrict__ dnew, int n) {
+                                                 const float* __restrict__ abuf, const float* __restrict__ anorm,
+                                                 int n) {
     const int pp = (int)perm[jj];
-    if (count[pp] > 0)
-        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)pp * n + t] = dnew[(long)(jj & 1) * n + t];
+    if (count[pp] > 0) {
+        const float nrm = du_anorm(jj, anorm);
+        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)pp * n + t] = abuf[(long)(jj & 1) * n + t] / nrm;
+    }
 }
 
 template <int NCH>
 __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
                                                    const int32_t* __restrict__ list_pos,
-                                                   const float* __restrict__ val, const float* __restrict__ X,
+                                                   const float* __restrict__ list_g, const float* __restrict__ X,
                                                    float* __restrict__ D, const float* __restrict__ R,
                                                    int32_t* __restrict__ unused, int32_t* __restrict__ usc,
                                                    int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
                                                    int N, int T, int M, float* __restrict__ apart,
-                                                   float* __restrict__ bpart, float* __restrict__ dnew,
-                                                   int32_t* __restrict__ ticket) {
+                                                   float* __restrict__ bpart, const float* __restrict__ abuf,
+                                                   const float* __restrict__ anorm, int32_t* __restrict__ ticket) {
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a
     __shared__ float rv[DU_WAVES];
     __shared__ int ri[DU_WAVES];
@@ -475,7 +491,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int G = gridDim.x;
     const bool vec = (n & 3) == 0;
-    if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, dnew, n);
+    if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, abuf, anorm, n);
     if (Np == 0) {
         // unused-atom special case (S/DictionaryUpdate.cu:238-302, SCupdate :406-427)
         const int counter = *usc;
@@ -546,7 +562,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     for (int base = gw; base < Np; base += 64 * GW) {
         const int mk = base + lane * GW;
         const int pos = (mk < Np) ? list_pos[beg + mk] : 0;
-        const float gl = (mk < Np) ? val[pos] : 0.f;
+        const float gl = (mk < Np) ? list_g[beg + mk] : 0.f;
         const int sl = pos / T;
         const int nk = min(64, (Np - base + GW - 1) / GW);
         for (int k = 0; k < nk; k += FQA) {
@@ -581,78 +597,72 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     }
 }
 
-// Sum of phase A's G block partials in a fixed order (one workgroup):
-// thread = (slice, group of 4 columns); each slice of partials is summed with
-// 8 independent accumulators (8 loads in flight), slices are combined in
-// order; then ||a|| and d_new = a / ||a|| (S/Multiplication.cu:221) into
-// dnew[jj & 1].  A separate launch (instead of a last-block ticket) keeps
-// device-scope fences out of phase A.
-template <int NCH>
+// Sum of phase A's G block partials, a = sum_b apart[b], in a fixed order,
+// spread over DU_RB workgroups by column: workgroup r owns float4 column
+// groups [r*ng4/DU_RB, (r+1)*ng4/DU_RB); thread = (column group, slice), slice
+// sl sums partials sl, sl+S, sl+2S, ...; slices are combined by a pairwise
+// tree in LDS.  Writes a into abuf[jj & 1] and workgroup r's sum of a^2 into
+// anorm[(jj & 1)*DU_RB + r]; consumers form ||a|| and d_new = a/||a||
+// (S/Multiplication.cu:221).  A separate launch instead of a last-block
+// ticket keeps device-scope fences out of phase A.
 __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const uint32_t* __restrict__ perm,
                                                           const int32_t* __restrict__ count,
-                                                          const float* __restrict__ apart, float* __restrict__ dnew,
-                                                          int n, int G) {
-    __shared__ float4 red[2048];       // slice sums, [slice][column group]
-    __shared__ float aval[256 * NCH];  // a
+                                                          const float* __restrict__ apart, float* __restrict__ abuf,
+                                                          float* __restrict__ anorm, int n, int G) {
+    __shared__ float4 red[512];
     __shared__ float wred[16];
     const int p = (int)perm[jj];
     if (count[p] == 0) return;
     const int np4 = (n + 3) & ~3;
     const int ng4 = np4 / 4;
-    const int nsl = max(1, min(8, (int)blockDim.x / ng4));
-    for (int g0 = 0; g0 < ng4; g0 += (int)blockDim.x / nsl) {
-        const int cg = g0 + (int)threadIdx.x % ((int)blockDim.x / nsl), sl = (int)threadIdx.x / ((int)blockDim.x / nsl);
-        if (sl < nsl && cg < ng4) {
-            const int b0 = G * sl / nsl, b1 = G * (sl + 1) / nsl;
-            float4 s8[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) s8[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-            int b = b0;
-            for (; b + 8 <= b1; b += 8) {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float4 v4 = *reinterpret_cast<const float4*>(apart + (long)(b + e) * np4 + 4 * cg);
-                    s8[e] = make_float4(s8[e].x + v4.x, s8[e].y + v4.y, s8[e].z + v4.z, s8[e].w + v4.w);
-                }
-            }
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                if (b + e < b1) {
-                    const float4 v4 = *reinterpret_cast<const float4*>(apart + (long)(b + e) * np4 + 4 * cg);
-                    s8[e] = make_float4(s8[e].x + v4.x, s8[e].y + v4.y, s8[e].z + v4.z, s8[e].w + v4.w);
-                }
-            }
-            float4 t4 = s8[0];
-#pragma unroll
-            for (int e = 1; e < 8; ++e) t4 = make_float4(t4.x + s8[e].x, t4.y + s8[e].y, t4.z + s8[e].z, t4.w + s8[e].w);
-            red[sl * ((int)blockDim.x / nsl) + (cg - g0)] = t4;
+    const int r = blockIdx.x;
+    const int c0 = r * ng4 / DU_RB, c1 = (r + 1) * ng4 / DU_RB;
+    int ncs = 1;
+    while (ncs < (ng4 + DU_RB - 1) / DU_RB) ncs *= 2;  // column-group slots (<= 16 for n <= 1024)
+    const int S = 512 / ncs;                            // slices per column group
+    const int cl = threadIdx.x % ncs, sl = threadIdx.x / ncs;
+    const int cg = c0 + cl;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cg < c1) {
+        for (int b = sl; b < G; b += S) {
+            const float4 v4 = *reinterpret_cast<const float4*>(apart + (long)b * np4 + 4 * cg);
+            acc = make_float4(acc.x + v4.x, acc.y + v4.y, acc.z + v4.z, acc.w + v4.w);
         }
-        __syncthreads();
-        const int gcount = (int)blockDim.x / nsl;
-        if ((int)threadIdx.x < gcount && g0 + (int)threadIdx.x < ng4) {
-            float4 t4 = red[threadIdx.x];
-            for (int q = 1; q < nsl; ++q) {
-                const float4 v4 = red[q * gcount + threadIdx.x];
-                t4 = make_float4(t4.x + v4.x, t4.y + v4.y, t4.z + v4.z, t4.w + v4.w);
-            }
-            reinterpret_cast<float4*>(aval)[g0 + threadIdx.x] = t4;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int half = 256; half >= ncs; half >>= 1) {  // pairwise tree over slices
+        if ((int)threadIdx.x < half) {
+            const float4 u = red[threadIdx.x], w4 = red[threadIdx.x + half];
+            red[threadIdx.x] = make_float4(u.x + w4.x, u.y + w4.y, u.z + w4.z, u.w + w4.w);
         }
         __syncthreads();
     }
     float sq = 0.f;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) sq += aval[t] * aval[t];
+    if ((int)threadIdx.x < ncs && cg < c1) {
+        const float4 a4 = red[threadIdx.x];
+        const float av[4] = {a4.x, a4.y, a4.z, a4.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int t = 4 * cg + e;
+            if (t < n) {
+                abuf[(long)(jj & 1) * n + t] = av[e];
+                sq += av[e] * av[e];
+            }
+        }
+    }
     sq = block_sum(sq, wred);
-    const float nrm = sqrtf(sq);
-    for (int t = threadIdx.x; t < n; t += blockDim.x) dnew[(long)(jj & 1) * n + t] = aval[t] / nrm;
+    if (threadIdx.x == 0) anorm[(jj & 1) * DU_RB + r] = sq;
 }
-
 
 template <int NCH>
 __global__ __launch_bounds__(512) void du_b_kernel(int jj, const uint32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
-                                                   const int32_t* __restrict__ list_pos, float* __restrict__ val,
+                                                   const int32_t* __restrict__ list_pos,
+                                                   const float* __restrict__ list_g, float* __restrict__ val,
                                                    const float* __restrict__ D, float* __restrict__ R,
-                                                   const float* __restrict__ dnew, int n, int T) {
+                                                   const float* __restrict__ abuf, const float* __restrict__ anorm,
+                                                   int n, int T) {
     const int p = (int)perm[jj];
     const int Np = count[p];
     if (Np == 0) return;
@@ -661,17 +671,19 @@ __global__ __launch_bounds__(512) void du_b_kernel(int jj, const uint32_t* __res
     const int beg = ptr[p];
     const int gw = blockIdx.x * DU_WAVES + w, GW = gridDim.x * DU_WAVES;
     const float* dp = D + (long)p * n;
-    const float* dq = dnew + (long)(jj & 1) * n;
+    const float* aq = abuf + (long)(jj & 1) * n;
+    const float nrm = du_anorm(jj, anorm);
     float4 dv[NCH], dn[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         dv[c] = ld4(dp, 256 * c + 4 * lane, n, vec);
-        dn[c] = ld4(dq, 256 * c + 4 * lane, n, vec);
+        const float4 a4 = ld4(aq, 256 * c + 4 * lane, n, vec);
+        dn[c] = make_float4(a4.x / nrm, a4.y / nrm, a4.z / nrm, a4.w / nrm);  // d_new = a / ||a||
     }
     for (int base = gw; base < Np; base += 64 * GW) {
         const int mk = base + lane * GW;
         const int pos = (mk < Np) ? list_pos[beg + mk] : 0;
-        const float gl = (mk < Np) ? val[pos] : 0.f;
+        const float gl = (mk < Np) ? list_g[beg + mk] : 0.f;
         const int sl = pos / T;
         const int nk = min(64, (Np - base + GW - 1) / GW);
         for (int k = 0; k < nk; k += FQ) {
@@ -716,22 +728,25 @@ __global__ __launch_bounds__(512) void du_b_kernel(int jj, const uint32_t* __res
 }
 
 __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const int32_t* __restrict__ count,
-                               float* __restrict__ D, const float* __restrict__ dnew, int n) {
-    du_commit_column(jj, perm, count, D, dnew, n);
+                               float* __restrict__ D, const float* __restrict__ abuf,
+                               const float* __restrict__ anorm, int n) {
+    du_commit_column(jj, perm, count, D, abuf, anorm, n);
 }
 
 template <int NCH>
 void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hipStream_t st) {
+    float* abuf = w.dnew;                 // 2 x n: a by sweep-position parity
+    float* anorm = w.dnew + 2 * w.n;      // 2 x DU_RB: partial sums of a^2
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
-                           w.list_pos, val, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T, M,
-                           w.apart, w.bpart, w.dnew, w.tickets);
-        hipLaunchKernelGGL((du_a_reduce_kernel<NCH>), dim3(1), dim3(512), 0, st, jj, w.perm_dev, w.count, w.apart,
-                           w.dnew, w.n, DU_BLOCKS);
+                           w.list_pos, w.list_g, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N,
+                           w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets);
+        hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj, w.perm_dev, w.count, w.apart, abuf,
+                           anorm, w.n, DU_BLOCKS);
         hipLaunchKernelGGL((du_b_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
-                           w.list_pos, val, D, w.R, w.dnew, w.n, w.T);
+                           w.list_pos, w.list_g, val, D, w.R, abuf, anorm, w.n, w.T);
     }
-    hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, w.dnew, w.n);
+    hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n);
 }
 
 // ---- error: S/KSVD.cu:235-262, S/ReductionKernels.cu:257-322 ---------------
@@ -960,19 +975,20 @@ hipError_t launch_du_init(DuWork& w, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* cnt, hipStream_t st) {
+hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* cnt, hipStream_t st,
+                                 const float* val) {
     hipLaunchKernelGGL(du_hist_kernel, dim3(w.nchunks), dim3(256), (size_t)w.K * 4, st, idx, cnt, w.N, w.T, w.K,
                        w.chunk, w.chunk_hist);
     hipLaunchKernelGGL(du_scan_chunks_kernel, dim3(w.K), dim3(256), 0, st, w.chunk_hist, w.nchunks, w.K, w.count);
     hipLaunchKernelGGL(du_scan_ptr_kernel, dim3(1), dim3(1024), 0, st, w.count, w.K, w.ptr);
     hipLaunchKernelGGL(du_fill_kernel, dim3(w.nchunks), dim3(64), (size_t)w.K * 4, st, idx, cnt, w.N, w.T, w.K,
-                       w.chunk, w.chunk_hist, w.ptr, w.list_pos);
+                       w.chunk, w.chunk_hist, w.ptr, w.list_pos, val, val ? w.list_g : nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* idx, float* val, const int32_t* cnt,
                            hipStream_t st) {
-    hipError_t e = launch_du_build_lists(w, idx, cnt, st);
+    hipError_t e = launch_du_build_lists(w, idx, cnt, st, val);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(du_residual_kernel, dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n, w.N, w.T, w.R);
     const int M = w.N < w.max_signals ? w.N : w.max_signals;
