@@ -733,6 +733,51 @@ __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const 
     du_commit_column(jj, perm, count, D, abuf, anorm, n);
 }
 
+// R = X - D Gamma for n <= 1024: one wave per signal, each lane owns 4
+// consecutive rows per 256 (float4), 4 dictionary columns in flight; the
+// signal's own row of X is fetched first so its latency overlaps the gathers.
+template <int NCH>
+__global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restrict__ X, const float* __restrict__ D,
+                                                           const int32_t* __restrict__ idx,
+                                                           const float* __restrict__ val,
+                                                           const int32_t* __restrict__ cnt, int n, int N, int T,
+                                                           float* __restrict__ R) {
+    const int lane = threadIdx.x & 63;
+    const bool vec = (n & 3) == 0;
+    for (long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6); s < N; s += (long)gridDim.x * 4) {
+        const int c = cnt[s];
+        const int il = (lane < c) ? idx[s * T + lane] : 0;
+        const float vl = (lane < c) ? val[s * T + lane] : 0.f;
+        float4 x4[NCH], xh[NCH];
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            x4[ch] = ld4(X + s * n, 256 * ch + 4 * lane, n, vec);
+            xh[ch] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        for (int j = 0; j < c; j += 4) {
+            float4 d4[4][NCH];
+            float g[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = j + u < c;
+                const long col = readlane(il, ok ? j + u : j);
+                g[u] = ok ? readlane(vl, j + u) : 0.f;
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch)
+                    d4[u][ch] = ok ? ld4(D + col * n, 256 * ch + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) xh[ch] = f4fma(g[u], d4[u][ch], xh[ch]);
+        }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+            st4(R + s * n, 256 * ch + 4 * lane, n, vec,
+                make_float4(x4[ch].x - xh[ch].x, x4[ch].y - xh[ch].y, x4[ch].z - xh[ch].z, x4[ch].w - xh[ch].w));
+    }
+}
+
 template <int NCH>
 void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hipStream_t st) {
     float* abuf = w.dnew;                 // 2 x n: a by sweep-position parity
@@ -990,15 +1035,25 @@ hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* i
                            hipStream_t st) {
     hipError_t e = launch_du_build_lists(w, idx, cnt, st, val);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(du_residual_kernel, dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n, w.N, w.T, w.R);
     const int M = w.N < w.max_signals ? w.N : w.max_signals;
     if (w.n <= 1024) {
         const int nch = (w.n + 255) / 256;
-        if (nch == 1) launch_du_atoms<1>(w, X, D, val, M, st);
-        else if (nch == 2) launch_du_atoms<2>(w, X, D, val, M, st);
-        else launch_du_atoms<4>(w, X, D, val, M, st);
+        if (nch == 1) {
+            hipLaunchKernelGGL((du_residual4_kernel<1>), dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n,
+                               w.N, w.T, w.R);
+            launch_du_atoms<1>(w, X, D, val, M, st);
+        } else if (nch == 2) {
+            hipLaunchKernelGGL((du_residual4_kernel<2>), dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n,
+                               w.N, w.T, w.R);
+            launch_du_atoms<2>(w, X, D, val, M, st);
+        } else {
+            hipLaunchKernelGGL((du_residual4_kernel<4>), dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n,
+                               w.N, w.T, w.R);
+            launch_du_atoms<4>(w, X, D, val, M, st);
+        }
         return hipGetLastError();
     }
+    hipLaunchKernelGGL(du_residual_kernel, dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n, w.N, w.T, w.R);
     const size_t lds_a = (size_t)4 * w.n * sizeof(float);
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL(du_phase_a_kernel, dim3(DU_BLOCKS), dim3(256), lds_a, st, jj, w.perm_dev, w.count, w.ptr,
