@@ -138,8 +138,8 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.list_pos, (size_t)c.N * c.T, "list_pos"));
     TRY(alloc(h, &w.list_g, (size_t)c.N * c.T, "list_g"));
     TRY(alloc(h, &w.R, (size_t)c.n * c.N, "R"));
-    TRY(alloc(h, &w.apart, (size_t)256 * ((c.n + 3) & ~3), "apart"));
-    TRY(alloc(h, &w.bpart, (size_t)2 * 256, "bpart"));
+    TRY(alloc(h, &w.apart, (size_t)ksvd::kDuBlocks * ((c.n + 3) & ~3), "apart"));
+    TRY(alloc(h, &w.bpart, (size_t)2 * ksvd::kDuBlocks, "bpart"));
     TRY(alloc(h, &w.dnew, (size_t)2 * c.n + 32, "dnew"));
     TRY(alloc(h, &w.scal, 8, "scal"));
     TRY(alloc(h, &w.tickets, 4, "tickets"));

@@ -197,6 +197,7 @@ hipError_t launch_compact_to_slotmap(const int32_t* idx, const float* val, const
                                      float* sel, float* c, hipStream_t st);
 
 // Dictionary update / error / clear (dict_update.hip)
+constexpr int kDuBlocks = 512;  // workgroups of the per-atom phase kernels (one block partial each)
 struct DuWork {
     int n, K, N, T, max_signals, use_thresh;
     double mu_thresh;
@@ -207,8 +208,8 @@ struct DuWork {
     int32_t* list_pos;   // N*T
     float* list_g;       // N*T: gamma of each list entry at list-build time
     float* R;            // n x N restricted residuals X - D Gamma
-    float* apart;        // DU_BLOCKS x (n rounded up to 4) partial a
-    float* bpart;        // DU_BLOCKS x 2 partial arg-max (special case)
+    float* apart;        // kDuBlocks x (n rounded up to 4) partial a
+    float* bpart;        // kDuBlocks x 2 partial arg-max (special case)
     float* dnew;         // 2 x n + 2 x 16: a and its partial sums of squares, by sweep-position parity
     float* scal;         // 8 scalars (norm etc.)
     int32_t* tickets;    // 4 last-block-done counters (zero between launches)

@@ -25,7 +25,7 @@ namespace ksvd {
 
 namespace {
 
-constexpr int DU_BLOCKS = 256;     // grid of the per-atom phase kernels (grid-stride over the atom's list)
+constexpr int DU_BLOCKS = kDuBlocks;  // grid of the per-atom phase kernels (grid-stride over the atom's list)
 constexpr int RES_BLOCKS = 2048;   // grid of the residual initialisation
 
 __device__ __forceinline__ void wave_lds_fence() {
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
 // and du_tail_kernel does it for the last atom of the sweep; nothing reads
 // d_p in between.
 constexpr int DU_WAVES = 8;
-constexpr int FQ = 4;   // residual rows in flight per wave (phase B: rows kept for the update)
+constexpr int FQ = 8;   // residual rows in flight per wave (phase B: rows kept for the update)
 constexpr int FQA = 8;  // residual rows in flight per wave (phase A)
 constexpr int DU_RB = 16;  // workgroups of the phase A reduction
 
