@@ -77,6 +77,57 @@ def cpu_baseline(X_sample, D, T, threads):
     return Xs.shape[0] / dt, used, dt
 
 
+def compare(args):
+    """The reference's end-to-end check (Matlab/fmriKSVD.m:26-70) on this
+    project's CPU path: the same X and random initial D through the FP32 port
+    of the algorithm (oracle/liboracle_f32.so, OpenMP) and through the GPU
+    engine; prints the final errors, their difference and the speedup."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from ksvd_amd import Engine, synth
+    from ksvd_amd import _lib
+
+    n, N, K, T = (int(v) for v in args.compare_shape.split(","))
+    if args.compare_x:
+        X = np.ascontiguousarray(_lib.mat_read(args.compare_x, "X").T, dtype=np.float32)  # (N, n)
+        N, n = X.shape
+    else:  # fMRI-shaped: AR(1) sources, T of them per signal, mean-removed unit columns
+        X = synth.make("C1", N=N, n=n, K=K, T=T).X.numpy()
+    D0 = np.random.default_rng(7).standard_normal((K, n)).astype(np.float32)  # D = randn(size(X,1), sources)
+    iters = args.compare_iters
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    o = oracle.load("f32")
+    used = o.set_threads(threads)
+    t0 = time.perf_counter()
+    cpu = o.ksvd(X, D0, T, iters, seed=0)
+    t_cpu = time.perf_counter() - t0
+    dev = torch.device("cuda", 0)
+    Xg, Dg = torch.from_numpy(X).to(dev), torch.from_numpy(D0).to(dev)
+    eng = Engine(n, K, N, T, perm_seed=0)
+    eng.ExecuteIterations(Xg, Dg, 1)  # warm-up (allocations, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, _, _, _, rmse = eng.ExecuteIterations(Xg, Dg, iters)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    eng.close()
+    e_cpu, e_gpu = float(cpu["rmse"][-1]), float(rmse[-1])
+    speedup = t_cpu / t_gpu
+    print(f">Time for k-SVD training in CPU: {t_cpu:.6f}", file=sys.stderr)
+    print(f">Time for k-SVD training in GPU: {t_gpu:.6f}", file=sys.stderr)
+    print(f">Final errors\n  CPU = {e_cpu:.5f}, GPU = {e_gpu:.5f}", file=sys.stderr)
+    print(f">Difference = {abs(e_cpu - e_gpu):.5f}", file=sys.stderr)
+    print(f">Speedup = {speedup:.2f} ({(1 - 1 / speedup) * 100:.2f} %)", file=sys.stderr)
+    print(json.dumps({"compare": {"n": n, "N": N, "K": K, "T": T, "iterations": iters,
+                                  "cpu_s": round(t_cpu, 6), "cpu_threads": used, "gpu_s": round(t_gpu, 6),
+                                  "rmse_cpu": e_cpu, "rmse_gpu": e_gpu, "difference": abs(e_cpu - e_gpu),
+                                  "speedup": round(speedup, 3),
+                                  "rmse_per_iter_cpu": [float(v) for v in cpu["rmse"]],
+                                  "rmse_per_iter_gpu": [float(v) for v in rmse]}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,7 +143,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run one device copy of GIB GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
+    ap.add_argument("--compare", action="store_true",
+                    help="only run the fMRI comparison: CPU (FP32 port) vs GPU K-SVD on one problem, then exit")
+    ap.add_argument("--compare-x", default=None, help="MAT-file with variable X for --compare (default: synthetic)")
+    ap.add_argument("--compare-shape", default="300,3600,9,6", help="n,N,K,T of the synthetic --compare problem")
+    ap.add_argument("--compare-iters", type=int, default=30)
     args = ap.parse_args()
+    if args.compare:
+        return compare(args)
     if args.copy_probe > 0:
         import torch
         x = torch.ones(int(args.copy_probe * (1 << 28)), device="cuda")

@@ -212,6 +212,26 @@ int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, f
  * seeded SplitMix64 Fisher-Yates shuffle of the same running buffer. */
 int ksvd_perm_sequence(uint64_t seed, int K, int iterations, uint32_t* out);
 
+/* ---- MAT-file I/O (host only, no device work) --------------------------- */
+
+/* Reads the real 2-D numeric variable `name` of a level-5 MAT-file (any numeric
+ * storage type, miCOMPRESSED or not) as column-major doubles.  With out == NULL
+ * only *rows / *cols are returned; otherwise capacity must be >= rows*cols.
+ * Replaces readMat (S/FileManager.cu:93-122: matOpen, matGetVariable, mxGetPr,
+ * mxGetM/N of the MATLAB MAT API). */
+int ksvd_mat_read(const char* path, const char* name, double* out, int64_t capacity, int64_t* rows,
+                  int64_t* cols);
+
+/* Writes a rows x cols column-major double matrix as variable `name`
+ * (uncompressed level 5), creating the file (append == 0) or adding to it
+ * (append != 0).  global != 0 sets the global flag as matPutVariableAsGlobal
+ * does.  Replaces writeMat (S/FileManager.cu:126-152). */
+int ksvd_mat_write(const char* path, const char* name, const double* data, int64_t rows, int64_t cols,
+                   int append, int global);
+
+/* Text of the last MAT-file error on the calling thread. */
+const char* ksvd_mat_last_error(void);
+
 /* ---- differentiation ----------------------------------------------------- */
 
 /* Adjoint of the fixed-support least squares gamma_I = (D_I^T D_I)^{-1} D_I^T x

@@ -26,7 +26,8 @@ EXPORTS = [
     "ksvd_batch_omp_compact", "ksvd_batch_omp_from_signals", "ksvd_batch_omp_from_signals_dense",
     "ksvd_batch_omp_eps", "ksvd_batch_omp_eps_from_signals", "ksvd_omp_stats", "ksvd_du_init",
     "ksvd_dictionary_update", "ksvd_dictionary_update_dense", "ksvd_error", "ksvd_clear_dict",
-    "ksvd_execute_iterations", "ksvd_perm_sequence", "ksvd_omp_adjoint",
+    "ksvd_execute_iterations", "ksvd_perm_sequence", "ksvd_omp_adjoint", "ksvd_mat_read", "ksvd_mat_write",
+    "ksvd_mat_last_error",
 ]
 
 
@@ -88,6 +89,9 @@ def load() -> C.CDLL:
     sig("ksvd_execute_iterations", I, H, P, P, P, P, P, P, I, P)
     sig("ksvd_perm_sequence", I, C.c_uint64, I, I, P)
     sig("ksvd_omp_adjoint", I, H, P, P, P, P, P, P, P, P)
+    sig("ksvd_mat_read", I, C.c_char_p, C.c_char_p, P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64))
+    sig("ksvd_mat_write", I, C.c_char_p, C.c_char_p, P, C.c_int64, C.c_int64, I, I)
+    sig("ksvd_mat_last_error", C.c_char_p)
     _lib = lib
     return lib
 
@@ -112,3 +116,36 @@ def default_config(**kw) -> Config:
             v = OMP_VARIANTS[v]
         setattr(cfg, k, v)
     return cfg
+
+
+def mat_read(path: str, name: str):
+    """Variable `name` of a level-5 MAT-file as a (rows, cols) float64 array
+    (host only; ksvd_mat_read)."""
+    import numpy as np
+    lib = load()
+    r, c = C.c_int64(), C.c_int64()
+    rc = lib.ksvd_mat_read(path.encode(), name.encode(), None, 0, C.byref(r), C.byref(c))
+    if rc != 0:
+        raise KsvdError(lib.ksvd_mat_last_error().decode())
+    out = np.empty((c.value, r.value), dtype=np.float64)  # column-major storage, transposed view below
+    rc = lib.ksvd_mat_read(path.encode(), name.encode(), out.ctypes.data_as(C.c_void_p), out.size, C.byref(r),
+                           C.byref(c))
+    if rc != 0:
+        raise KsvdError(lib.ksvd_mat_last_error().decode())
+    return out.T
+
+
+def mat_write(path: str, variables: dict, global_vars: bool = False):
+    """Writes {name: 2-D array} as an uncompressed level-5 MAT-file of doubles
+    (host only; ksvd_mat_write)."""
+    import numpy as np
+    lib = load()
+    for i, (name, a) in enumerate(variables.items()):
+        a = np.asarray(a, dtype=np.float64)
+        if a.ndim != 2:
+            raise ValueError(f"{name}: MAT variables here are 2-D")
+        col = np.ascontiguousarray(a.T)  # column-major bytes
+        rc = lib.ksvd_mat_write(path.encode(), name.encode(), col.ctypes.data_as(C.c_void_p), a.shape[0], a.shape[1],
+                                1 if i else 0, 1 if global_vars else 0)
+        if rc != 0:
+            raise KsvdError(lib.ksvd_mat_last_error().decode())

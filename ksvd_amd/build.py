@@ -13,7 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libksvd_amd.so")
-SOURCES = ["gemm.hip", "omp.hip", "omp_qform.hip", "omp_fast.hip", "dict_update.hip", "adjoint.hip", "capi.cpp"]
+SOURCES = ["gemm.hip", "omp.hip", "omp_qform.hip", "omp_fast.hip", "dict_update.hip", "adjoint.hip", "capi.cpp",
+           "matio.cpp"]
+MAIN = os.path.join(HERE, "ksvd_main")  # CLI driver (ksvd_main.cpp) linked against the library
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KSVD_ARCH", "gfx950")
@@ -40,11 +42,18 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    tmp = MAIN + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "ksvd_main.cpp"), "-o", tmp, f"-L{HERE}",
+           "-lksvd_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ksvd_main build failed:\n{r.stderr}")
+    os.replace(tmp, MAIN)
     if verbose:
         print(f"built {LIB}")
     return LIB

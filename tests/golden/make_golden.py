@@ -1,9 +1,13 @@
 """Writes the golden fixtures under tests/golden/ (run in the build container).
 
 Reads the reference's own data files as DATA (scipy.io.loadmat, no code runs):
-  K-SVD/K-SVD/input/D.mat                      -> ref_D.npy (300 x 9 float64)
+  K-SVD/K-SVD/input/D.mat                      -> ref_D.npy (300 x 9 float64),
+                                                  and the file itself as ref_D.mat
   K-SVD/K-SVD/output/temp.mat,
-  K-SVD/x64/Release/output/temp.mat (CUDA_Gamma) -> ref_gamma_structure.json
+  K-SVD/x64/Release/output/temp.mat (CUDA_Gamma) -> ref_gamma_structure.json, and
+                                                  the first file itself as ref_temp.mat
+The two .mat files are kept byte for byte (MATLAB-written, miCOMPRESSED) to
+test the engine's own MAT reader against scipy's.
 The reference's X.mat is not in the tree, so CUDA_Gamma's values cannot be
 reproduced; only its structure (shape, nonzeros per column and per row) is kept.
 
@@ -65,6 +69,9 @@ def main():
     import scipy.io as sio
     D = sio.loadmat(os.path.join(REF, "K-SVD/K-SVD/input/D.mat"))["D"]
     np.save(os.path.join(HERE, "ref_D.npy"), np.ascontiguousarray(D, dtype=np.float64))
+    import shutil
+    shutil.copyfile(os.path.join(REF, "K-SVD/K-SVD/input/D.mat"), os.path.join(HERE, "ref_D.mat"))
+    shutil.copyfile(os.path.join(REF, "K-SVD/K-SVD/output/temp.mat"), os.path.join(HERE, "ref_temp.mat"))
     st = {}
     for key, rel in [("K-SVD/K-SVD/output/temp.mat", "K-SVD/K-SVD/output/temp.mat"),
                      ("K-SVD/x64/Release/output/temp.mat", "K-SVD/x64/Release/output/temp.mat")]:
@@ -77,6 +84,8 @@ def main():
             "finite": bool(np.isfinite(G).all()),
             "abs_max": float(np.abs(G).max()),
             "abs_min_nonzero": float(np.abs(G[nz]).min()),
+            "sum": float(G.sum()),
+            "abs_sum": float(np.abs(G).sum()),
         }
     with open(os.path.join(HERE, "ref_gamma_structure.json"), "w") as f:
         json.dump(st, f, indent=1)
