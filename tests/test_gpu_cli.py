@@ -65,3 +65,19 @@ def test_driver_end_to_end(o64, tmp_path):
     FD = sio.loadmat(str(tmp_path / "output" / "D.mat"))["FinalD"]
     np.testing.assert_array_equal(FD, Dg.cpu().numpy().T.astype(np.float64))
     eng.close()
+
+
+def test_compare_mode():
+    """bench.py --compare (the fmriKSVD.m check, Matlab/fmriKSVD.m:26-70): CPU FP32 port
+    and GPU engine on the same problem end at the same RMSE, iteration by iteration."""
+    import json
+    import sys
+    say("bench.py --compare 300x3600 K=9 T=6, 10 iterations")
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--compare", "--compare-iters", "10"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    c = json.loads(r.stdout.strip().splitlines()[-1])["compare"]
+    assert (c["n"], c["N"], c["K"], c["T"], c["iterations"]) == (300, 3600, 9, 6, 10)
+    assert ">Difference = " in r.stderr and ">Speedup = " in r.stderr
+    np.testing.assert_allclose(c["rmse_per_iter_gpu"], c["rmse_per_iter_cpu"], rtol=1e-4)
+    assert c["difference"] < 1e-5
