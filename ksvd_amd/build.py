@@ -27,10 +27,17 @@ FLAGS = [
 ]
 
 
+# Per-source extra flags.  The register-basis Batch-OMP kernel unrolls all of
+# its (up to 64) steps so that every index into its register-held basis is a
+# compile-time constant; LLVM's default size limit for `#pragma unroll` would
+# leave the loop rolled and push the basis to scratch memory.
+EXTRA = {"omp_qform.hip": ["-mllvm", "-pragma-unroll-threshold=500000"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     lang = ["-x", "hip"]
-    cmd = [HIPCC, *lang, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *lang, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

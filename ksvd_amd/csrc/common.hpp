@@ -92,6 +92,25 @@ __device__ __forceinline__ float wave_sum_dpp(float v) {
     return (r0 + r1) + (r2 + r3);
 }
 
+// Sum over the first 16*NR lanes (NR = 1, 2 or 4 rows of 16 lanes): DPP
+// within each row as in wave_sum_dpp, then the row sums in fixed order.
+// Deterministic; uniform control flow only.
+template <int NR>
+__device__ __forceinline__ float lanes_sum_dpp(float v) {
+    static_assert(NR == 1 || NR == 2 || NR == 4, "rows of 16 lanes");
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    if constexpr (NR == 1) return r0;
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    if constexpr (NR == 2) return r0 + r1;
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
+}
+
 // Wave max without LDS traffic: quad swaps (quad_perm [1,0,3,2], [2,3,0,1]),
 // then row_half_mirror and row_mirror make every 16-lane row uniform; the four
 // row maxima are combined from v_readlane.  Call from uniform control flow.

@@ -66,7 +66,7 @@ constexpr int qform_waves_per_eu(int kpl, int tt) {
 template <int KPL, int TT, int W, bool EPS>
 __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform_kernel(OmpArgs a) {
     constexpr int LD = TT + 1;
-    constexpr int UG = 4;        // steps per unrolled group (TT is a multiple)
+    constexpr int UG = 4;        // rows of Q per group (one float4 of w)
     constexpr int NG = TT / UG;  // groups of UG support slots
     __shared__ float Ls[TT * LD];  // L (row r, col c) at Ls[c*LD + r]
     __shared__ float ys[TT];       // y = L^-1 alpha0_I
@@ -113,11 +113,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
     // columns 0 .. TT-2 of Q: the column of step TT-1 (necessarily the last
     // step, T <= TT) is never read
     constexpr int QT = TT - 1;
-    float Q[QT][KPL];
-#pragma unroll
-    for (int j = 0; j < QT; ++j)
-#pragma unroll
-        for (int m = 0; m < KPL; ++m) Q[j][m] = 0.f;
+    float Q[QT][KPL];  // column j written at step j, read only by later steps
     for (int j = threadIdx.x; j < 2 * W * TT; j += 64 * W) (&wrow[0][0][0])[j] = 0.f;
     for (int j = threadIdx.x; j < W * TT; j += 64 * W) (&wfix[0][0])[j] = 0.f;
     Support sp;
@@ -130,22 +126,20 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
     int brk = -1;  // atom marked in SELECTED_ATOMS by a pivot break (S/OMP.cu:313,331-333)
     __syncthreads();
 
-    // Steps i = UG*c + u: u is a compile-time constant (inner loop unrolled),
-    // so indexing Q by i costs an NG-long chain over c instead of TT.
+    // Steps fully unrolled: i is a compile-time constant, so every index into
+    // Q and every loop over the support is static (no compare chains).
     int cnt = 0;
-    bool go = true;
-    for (int c = 0; go && c < NG; ++c) {
 #pragma unroll
-        for (int u = 0; u < UG; ++u) {
-            const int i = UG * c + u;
+    for (int i = 0; i < TT; ++i) {
+        {
             // groups of UG rows of Q holding slots < i
-            const int ng = c + (u > 0 ? 1 : 0);
-            if (i >= T) { go = false; break; }
+            const int ng = (i + UG - 1) / UG;
+            if (i >= T) break;
             // error mode: ||x - D_I gamma_I||^2 = ||x||^2 - ||y||^2
             if constexpr (EPS) {
-                if (xn2 - ysum <= e2) { go = false; break; }
+                if (xn2 - ysum <= e2) break;
             }
-            const int b = u & 1;  // step parity (UG*c is even; constant after unrolling)
+            const int b = i & 1;  // step parity
             // this wave's arg-max of alpha^2: largest value, lowest atom on ties
             // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges).
             // alpha^2 >= 0, so its bit pattern orders like a signed int.
@@ -173,8 +167,10 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
                         for (int g = 0; g < NG; ++g) {
                             if (g < ng) {
 #pragma unroll
-                                for (int e = 0; e < UG; ++e)
-                                    if (UG * g + e < QT) wrow[b][v][UG * g + e] = Q[UG * g + e][m];
+                                for (int e = 0; e < UG; ++e) {
+                                    const int j = UG * g + e;
+                                    wrow[b][v][j] = (j < i && j < QT) ? Q[j < QT ? j : 0][m] : 0.f;
+                                }
                             }
                         }
                     }
@@ -196,47 +192,23 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
             const float bestf = __int_as_float(uniform(best));
             // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
             // pos >= K: only padding atoms left (every real alpha NaN or 0)
-            if ((unsigned)pos >= (unsigned)K || !((double)bestf >= 1e-14)) { go = false; break; }
-            if (__ballot(lane < i && sp.ind == pos) != 0ull) { go = false; break; }
+            if ((unsigned)pos >= (unsigned)K || !((double)bestf >= 1e-14)) break;
+            if (__ballot(lane < i && sp.ind == pos) != 0ull) break;
             float g[KPL];
             load_slice<KPL>(G + (long)pos * K, kb, K, vec, g);
-            const float apos = cand_a[b][vo];
-            // w_j = q_j(pos): d = 1 - ||w||^2 and acc = Q w in one pass
+            // w_j = q_j(pos), one per lane j < i; d = 1 - ||w||^2 from a
+            // lane-parallel sum
             const float* wsrc = wrow[b][vo];
-            float acc[KPL];
-#pragma unroll
-            for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
-            float ss = 0.f;
-#pragma unroll
-            for (int gg = 0; gg < NG; ++gg) {
-                if (gg < ng) {
-#pragma unroll
-                    for (int h = 0; h < UG / 4; ++h) {
-                        const float4 w4 = *reinterpret_cast<const float4*>(wsrc + UG * gg + 4 * h);
-                        const int j = UG * gg + 4 * h;
-                        ss = fmaf(w4.x, w4.x, ss);
-                        ss = fmaf(w4.y, w4.y, ss);
-                        ss = fmaf(w4.z, w4.z, ss);
-                        ss = fmaf(w4.w, w4.w, ss);
-                        const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            if (j + e < QT) {
-#pragma unroll
-                                for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j + e][m], wj[e], acc[m]);
-                            }
-                    }
-                }
-            }
-            const float d = 1.f - ss;
+            float wv = (lane < i) ? wsrc[lane & (TT - 1)] : 0.f;  // row i of L (written by wave 0)
+            const float d = 1.f - lanes_sum_dpp<(TT + 15) / 16>(wv * wv);
             double dd = (double)d;
             const bool redo = i > 0 && dd < (double)a.pivot_fp64;
-            float wv = 0.f;  // w_lane: row i of L (written to Ls by wave 0)
             if (redo) {
                 // ill-conditioned: w = L^-1 G(I, pos) and d again in FP64
                 // (backsubt_indexed_append_L / sumSquareValues, S/OMP.cu:169-182, 221-228)
                 n_redo++;
                 double b64 = (lane < i) ? (double)G[(long)pos * K + sp.ind] : 0.0, w64 = 0.0;
+#pragma unroll 1
                 for (int j = 0; j < i; ++j) {
                     const double wj = readlane(b64, j) / (double)Ls[j * LD + j];
                     if (lane == j) w64 = wj;
@@ -246,35 +218,15 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
                 wv = (float)w64;
                 if (lane < i) wfix[v][lane] = wv;
                 wave_lds_fence();
-#pragma unroll
-                for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
-#pragma unroll
-                for (int gg = 0; gg < NG; ++gg) {
-                    if (gg < ng) {
-#pragma unroll
-                        for (int h = 0; h < UG / 4; ++h) {
-                            const float4 w4 = *reinterpret_cast<const float4*>(&wfix[v][UG * gg + 4 * h]);
-                            const int j = UG * gg + 4 * h;
-                            const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-                            for (int e = 0; e < 4; ++e)
-                                if (j + e < QT) {
-#pragma unroll
-                                    for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j + e][m], wj[e], acc[m]);
-                                }
-                        }
-                    }
-                }
-            } else if (v == 0) {
-                wv = (lane < i) ? wsrc[lane & (TT - 1)] : 0.f;
+                wsrc = wfix[v];
             }
             if (dd <= 1e-14) {  // S/OMP.cu:331-333
                 brk = pos;
-                go = false;
                 break;
             }
-            // q_i = (G(:, pos) - Q w) / l_ii,  alpha -= q_i y_i.  1/l_ii from
-            // v_rsq_f32 (1 ulp) on the FP32 pivot, from FP64 after a recompute
+            // l_ii, 1/l_ii and y_i, and row i of L, before the Q w pass (w_lane
+            // and the pivot are dead during it).  1/l_ii from v_rsq_f32 (1 ulp)
+            // on the FP32 pivot, from FP64 after a recompute
             float rl, l;
             if (redo) {
                 const double sq = sqrt(dd);
@@ -284,28 +236,44 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform
                 rl = __builtin_amdgcn_rsqf(d);
                 l = d * rl;
             }
-            const float y = apos * rl;
+            const float y = cand_a[b][vo] * rl;
+            if (v == 0) {
+                if (lane < i) Ls[lane * LD + i] = wv;
+                if (lane == i) { Ls[i * LD + i] = l; ys[i] = y; }
+            }
+            if constexpr (EPS) ysum += (double)y * (double)y;
+            // acc = Q w (rows j < i)
+            float acc[KPL];
+#pragma unroll
+            for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
+#pragma unroll
+            for (int gg = 0; gg < NG; ++gg) {
+                if (gg < ng) {
+                    const float4 w4 = *reinterpret_cast<const float4*>(wsrc + UG * gg);
+                    const int j = UG * gg;
+                    const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (j + e < i && j + e < QT) {
+#pragma unroll
+                            for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j + e][m], wj[e], acc[m]);
+                        }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            // q_i = (G(:, pos) - Q w) / l_ii,  alpha -= q_i y_i
 #pragma unroll
             for (int m = 0; m < KPL; ++m) {
                 const float q = (g[m] - acc[m]) * rl;
                 al[m] = al[m] - q * y;
                 acc[m] = q;
             }
-            // Q(:, UG*c + u) = q
+            // Q(:, i) = q
+            if (i < QT) {
 #pragma unroll
-            for (int cc = 0; cc < NG; ++cc)
-                if (cc == c) {
-                    if (UG * cc + u < QT) {
-#pragma unroll
-                        for (int m = 0; m < KPL; ++m) Q[UG * cc + u][m] = acc[m];
-                    }
-                }
-            if (lane == i) sp.ind = pos;
-            if (v == 0) {
-                if (lane < i) Ls[lane * LD + i] = wv;
-                if (lane == i) { Ls[i * LD + i] = l; ys[i] = y; }
+                for (int m = 0; m < KPL; ++m) Q[i < QT ? i : 0][m] = acc[m];
             }
-            if constexpr (EPS) ysum += (double)y * (double)y;
+            if (lane == i) sp.ind = pos;
             cnt = i + 1;
         }
     }
