@@ -281,7 +281,7 @@ def main():
         "peak": FP32_PEAK_TF,
         "unit": "TFLOP/s",
         "frac": round(achieved_tf / FP32_PEAK_TF, 4),
-        "traffic": profiled_traffic("omp_qform.hip") if args.omp == "qform" else None,
+        "traffic": profiled_traffic("omp_qform.hpp") if args.omp == "qform" else None,
         "kernel": kname,
         "kernel_ms": round(omp_avg, 3),
         "work_per_launch": f"F_omp = K T(T+1) + T(T+1)(2T+1)/2 = {omp_flops_per_signal(K, T):.0f} FP32 FLOP per "

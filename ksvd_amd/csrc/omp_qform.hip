@@ -1,338 +1,14 @@
-// omp_qform.hip -- Batch-OMP with the Gram-domain basis held in registers.
-//
-// Same selections and results as parOMP (S/OMP.cu:254-405), computed through
-// a different but algebraically identical recurrence.  With the Cholesky
-// factor L L^T = G_II of the support and y = L^-1 alpha0_I, the reference
-// forms gamma = L^-T y and alpha = alpha0 - G(:,I) gamma every step (a K x i
-// product read from G).  Writing Q = G(:,I) L^-T (K x i), column j of Q is
-// fixed once atom j is in the support, and
-//   alpha^(i)   = alpha^(i-1) - q_i y_i
-//   w           = L^-1 G(I, k_i) = (q_0(k_i), ..., q_{i-1}(k_i))
-//   l_ii        = sqrt(1 - ||w||^2)
-//   q_i         = (G(:, k_i) - sum_j q_j w_j) / l_ii
-//   y_i         = alpha^(i-1)(k_i) / l_ii
-// so a step needs one column of G, the i columns of Q it already holds, and
-// no triangular solve; gamma = L^-T y is solved once at the end.
-//
-// Layout: one workgroup of W waves per signal.  Wave v, lane l holds atoms
-// k = (v*64 + l)*KPL + m, m < KPL, contiguous per lane (vector loads of the
-// DtX row and of G columns), with alpha[m] and Q[j][m] (j < TT-1) in VGPRs:
-// TT*KPL <= 128 registers, the per-signal working set T*K*4 bytes spread over
-// the workgroup's register files instead of being re-streamed from L2.
-// Per step, one barrier: each wave's arg-max (DPP max + ballot) and its
-// winning lane's row of Q go to LDS (double-buffered by step parity) ->
-// barrier -> every wave reduces the W candidates (same data, same order:
-// identical decisions, so every wave leaves the loop at the same step), loads
-// its slice of G(:, k_i), reads w from the winner's row and forms q_i.
-//
-// The pivot d = 1 - ||w||^2 is recomputed in FP64 from G(I, k_i) and L when
-// it falls below pivot_fp64, as in the wave kernels.
+// omp_qform.hip -- shape choice and dispatch of the register-basis Batch-OMP
+// kernel (omp_qform.hpp); one translation unit per basis size TT
+// (omp_qform_t{8,16,32,64}.hip) holds its instantiations.
 #include "omp_common.hpp"
 
 namespace ksvd {
 
-namespace {
-
-// out[m] = p[kb + m] for kb + m < K, else 0.  `vec`: 16-byte (or KPL*4-byte)
-// aligned vector loads are legal for this row (K a multiple of min(KPL, 4)).
-template <int KPL>
-__device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, int K, bool vec, float* out) {
-    if (vec && kb + KPL <= K) {
-        if constexpr (KPL == 1) {
-            out[0] = p[kb];
-        } else if constexpr (KPL == 2) {
-            const float2 u = *reinterpret_cast<const float2*>(p + kb);
-            out[0] = u.x; out[1] = u.y;
-        } else {
-#pragma unroll
-            for (int q = 0; q < KPL / 4; ++q) {
-                const float4 u = *reinterpret_cast<const float4*>(p + kb + 4 * q);
-                out[4 * q] = u.x; out[4 * q + 1] = u.y; out[4 * q + 2] = u.z; out[4 * q + 3] = u.w;
-            }
-        }
-    } else {
-#pragma unroll
-        for (int m = 0; m < KPL; ++m) out[m] = (kb + m < K) ? p[kb + m] : 0.f;
-    }
-}
-
-// Occupancy asked of the compiler for the largest bases (Q = 128 VGPRs):
-// Q, three KPL-wide arrays (alpha, the G column, Q w) and ~26 others must
-// fit 512/waves registers.  Smaller bases are left to the compiler.
-constexpr int qform_waves_per_eu(int kpl, int tt) {
-    return tt * kpl < 128 ? 1 : 512 / (tt * kpl + 3 * kpl + 26);
-}
-
-template <int KPL, int TT, int W, bool EPS>
-__global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT)) void omp_qform_kernel(OmpArgs a) {
-    constexpr int LD = TT + 1;
-    constexpr int UG = 4;        // rows of Q per group (one float4 of w)
-    constexpr int NG = TT / UG;  // groups of UG support slots
-    __shared__ float Ls[TT * LD];  // L (row r, col c) at Ls[c*LD + r]
-    __shared__ float ys[TT];       // y = L^-1 alpha0_I
-    // per step parity: each wave's candidate (alpha^2, atom, signed alpha) and
-    // the candidate's row of Q (zero beyond the step), which is w if it wins
-    __shared__ int cand_v[2][W], cand_k[2][W];
-    __shared__ float cand_a[2][W];
-    __shared__ __align__(16) float wrow[2][W][TT];
-    __shared__ __align__(16) float wfix[W][TT];  // per wave: w recomputed in FP64 (zero beyond the step)
-    extern __shared__ unsigned char slot[];      // K bytes, dense output only
-    const int lane = threadIdx.x & 63;
-    const int v = threadIdx.x >> 6;
-    const int K = a.K, T = a.T;
-    const long s = blockIdx.x;
-    const int kb = (v * 64 + lane) * KPL;
-    const float* __restrict__ G = a.G;
-    const bool vec = (K % (KPL < 4 ? KPL : 4)) == 0;
-
-    // alpha of this lane's atoms (0 for padding atoms k >= K: they sit above
-    // every real atom, so with the lowest-index tie rule they never win
-    // against one)
-    float al[KPL];
-    {
-        // the DtX row is read once: non-temporal, so it does not evict G
-        const float* __restrict__ p = a.DtX + s * (long)K;
-        if (vec && kb + KPL <= K) {
-            if constexpr (KPL == 1) {
-                al[0] = __builtin_nontemporal_load(p + kb);
-            } else if constexpr (KPL == 2) {
-                al[0] = __builtin_nontemporal_load(p + kb);
-                al[1] = __builtin_nontemporal_load(p + kb + 1);
-            } else {
-#pragma unroll
-                for (int q = 0; q < KPL / 4; ++q) {
-                    const f32x4 u = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p + kb + 4 * q));
-                    al[4 * q] = u[0]; al[4 * q + 1] = u[1]; al[4 * q + 2] = u[2]; al[4 * q + 3] = u[3];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < KPL; ++m) al[m] = (kb + m < K) ? p[kb + m] : 0.f;
-        }
-    }
-    // columns 0 .. TT-2 of Q: the column of step TT-1 (necessarily the last
-    // step, T <= TT) is never read
-    constexpr int QT = TT - 1;
-    float Q[QT][KPL];  // column j written at step j, read only by later steps
-    for (int j = threadIdx.x; j < 2 * W * TT; j += 64 * W) (&wrow[0][0][0])[j] = 0.f;
-    for (int j = threadIdx.x; j < W * TT; j += 64 * W) (&wfix[0][0])[j] = 0.f;
-    Support sp;
-    double xn2 = 0.0, e2 = 0.0, ysum = 0.0;
-    if constexpr (EPS) {
-        xn2 = (double)a.xnorm2[s];
-        e2 = (double)a.eps2[s];
-    }
-    unsigned n_redo = 0;
-    int brk = -1;  // atom marked in SELECTED_ATOMS by a pivot break (S/OMP.cu:313,331-333)
-    __syncthreads();
-
-    // Steps fully unrolled: i is a compile-time constant, so every index into
-    // Q and every loop over the support is static (no compare chains).
-    int cnt = 0;
-#pragma unroll
-    for (int i = 0; i < TT; ++i) {
-        {
-            // groups of UG rows of Q holding slots < i
-            const int ng = (i + UG - 1) / UG;
-            if (i >= T) break;
-            // error mode: ||x - D_I gamma_I||^2 = ||x||^2 - ||y||^2
-            if constexpr (EPS) {
-                if (xn2 - ysum <= e2) break;
-            }
-            const int b = i & 1;  // step parity
-            // this wave's arg-max of alpha^2: largest value, lowest atom on ties
-            // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges).
-            // alpha^2 >= 0, so its bit pattern orders like a signed int.
-            int bv = -1, bm = 0;
-#pragma unroll
-            for (int m = 0; m < KPL; ++m) {
-                const int c2 = __float_as_int(al[m] * al[m]);
-                if (c2 > bv) { bv = c2; bm = m; }
-            }
-            const int M = wave_max_i32_dpp(bv);
-            const int wl = __builtin_ctzll(__ballot(bv == M));
-            if (lane == wl) {
-                float av = al[0];
-#pragma unroll
-                for (int m = 1; m < KPL; ++m)
-                    if (m == bm) av = al[m];
-                cand_v[b][v] = M;
-                cand_k[b][v] = kb + bm;
-                cand_a[b][v] = av;
-                // rows j < i of Q for atom kb + bm (groups of UG; rows >= i are zero)
-#pragma unroll
-                for (int m = 0; m < KPL; ++m) {
-                    if (m == bm) {
-#pragma unroll
-                        for (int g = 0; g < NG; ++g) {
-                            if (g < ng) {
-#pragma unroll
-                                for (int e = 0; e < UG; ++e) {
-                                    const int j = UG * g + e;
-                                    wrow[b][v][j] = (j < i && j < QT) ? Q[j < QT ? j : 0][m] : 0.f;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-
-            // the block's arg-max over the W wave winners: strict '>' in wave
-            // order keeps the lowest atom among equal values
-            int best = cand_v[b][0];
-            int pos = cand_k[b][0], vo = 0;
-#pragma unroll
-            for (int w2 = 1; w2 < W; ++w2) {
-                const int cv = cand_v[b][w2];
-                if (cv > best) { best = cv; pos = cand_k[b][w2]; vo = w2; }
-            }
-            pos = uniform(pos);
-            vo = uniform(vo);
-            const float bestf = __int_as_float(uniform(best));
-            // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
-            // pos >= K: only padding atoms left (every real alpha NaN or 0)
-            if ((unsigned)pos >= (unsigned)K || !((double)bestf >= 1e-14)) break;
-            if (__ballot(lane < i && sp.ind == pos) != 0ull) break;
-            float g[KPL];
-            load_slice<KPL>(G + (long)pos * K, kb, K, vec, g);
-            // w_j = q_j(pos), one per lane j < i; d = 1 - ||w||^2 from a
-            // lane-parallel sum
-            const float* wsrc = wrow[b][vo];
-            float wv = (lane < i) ? wsrc[lane & (TT - 1)] : 0.f;  // row i of L (written by wave 0)
-            const float d = 1.f - lanes_sum_dpp<(TT + 15) / 16>(wv * wv);
-            double dd = (double)d;
-            const bool redo = i > 0 && dd < (double)a.pivot_fp64;
-            if (redo) {
-                // ill-conditioned: w = L^-1 G(I, pos) and d again in FP64
-                // (backsubt_indexed_append_L / sumSquareValues, S/OMP.cu:169-182, 221-228)
-                n_redo++;
-                double b64 = (lane < i) ? (double)G[(long)pos * K + sp.ind] : 0.0, w64 = 0.0;
-#pragma unroll 1
-                for (int j = 0; j < i; ++j) {
-                    const double wj = readlane(b64, j) / (double)Ls[j * LD + j];
-                    if (lane == j) w64 = wj;
-                    if (lane > j && lane < i) b64 = b64 - (double)Ls[j * LD + lane] * wj;
-                }
-                dd = uniform(1.0 - wave_sum((lane < i) ? w64 * w64 : 0.0));
-                wv = (float)w64;
-                if (lane < i) wfix[v][lane] = wv;
-                wave_lds_fence();
-                wsrc = wfix[v];
-            }
-            if (dd <= 1e-14) {  // S/OMP.cu:331-333
-                brk = pos;
-                break;
-            }
-            // l_ii, 1/l_ii and y_i, and row i of L, before the Q w pass (w_lane
-            // and the pivot are dead during it).  1/l_ii from v_rsq_f32 (1 ulp)
-            // on the FP32 pivot, from FP64 after a recompute
-            float rl, l;
-            if (redo) {
-                const double sq = sqrt(dd);
-                l = (float)sq;
-                rl = (float)(1.0 / sq);
-            } else {
-                rl = __builtin_amdgcn_rsqf(d);
-                l = d * rl;
-            }
-            const float y = cand_a[b][vo] * rl;
-            if (v == 0) {
-                if (lane < i) Ls[lane * LD + i] = wv;
-                if (lane == i) { Ls[i * LD + i] = l; ys[i] = y; }
-            }
-            if constexpr (EPS) ysum += (double)y * (double)y;
-            // acc = Q w (rows j < i)
-            float acc[KPL];
-#pragma unroll
-            for (int m = 0; m < KPL; ++m) acc[m] = 0.f;
-#pragma unroll
-            for (int gg = 0; gg < NG; ++gg) {
-                if (gg < ng) {
-                    const float4 w4 = *reinterpret_cast<const float4*>(wsrc + UG * gg);
-                    const int j = UG * gg;
-                    const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (j + e < i && j + e < QT) {
-#pragma unroll
-                            for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j + e][m], wj[e], acc[m]);
-                        }
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-            // q_i = (G(:, pos) - Q w) / l_ii,  alpha -= q_i y_i
-#pragma unroll
-            for (int m = 0; m < KPL; ++m) {
-                const float q = (g[m] - acc[m]) * rl;
-                al[m] = al[m] - q * y;
-                acc[m] = q;
-            }
-            // Q(:, i) = q
-            if (i < QT) {
-#pragma unroll
-                for (int m = 0; m < KPL; ++m) Q[i < QT ? i : 0][m] = acc[m];
-            }
-            if (lane == i) sp.ind = pos;
-            cnt = i + 1;
-        }
-    }
-    const int i = cnt;
-
-    if (v == 0) {
-        wave_lds_fence();
-        // gamma = L^-T y, lane-parallel backward sweep (backsubst_Lt, S/OMP.cu:126-138)
-        float rhs = (lane < i) ? ys[lane & (TT - 1)] : 0.f;
-        for (int r = i - 1; r >= 0; --r) {
-            const float gr = readlane(rhs, r) / Ls[r * LD + r];
-            if (lane == r) sp.gam = gr;
-            if (lane < r) rhs = rhs - Ls[lane * LD + r] * gr;
-        }
-        if (a.idx && lane < T) {
-            a.idx[s * T + lane] = (lane < i) ? sp.ind : -1;
-            a.val[s * T + lane] = (lane < i) ? sp.gam : 0.f;
-        }
-        if (a.cnt && lane == 0) a.cnt[s] = i;
-        if (a.c && lane < i) a.c[s * T + lane] = sp.gam;
-        if (a.stats && lane == 0 && n_redo) atomicAdd(a.stats + 1, (unsigned long long)n_redo);
-    }
-    if (a.sel) {
-        // SELECTED_ATOMS column: slot+1 of each support atom, and the atom a
-        // pivot break marked before stopping
-        for (int k = threadIdx.x; k < K; k += 64 * W) slot[k] = 0;
-        __syncthreads();
-        if (v == 0) {
-            if (lane < i) slot[sp.ind] = (unsigned char)(lane + 1);
-            if (lane == 0 && brk >= 0) slot[brk] = (unsigned char)(i + 1);
-        }
-        __syncthreads();
-        float* selp = a.sel + s * (long)K;
-        for (int k = threadIdx.x; k < K; k += 64 * W) selp[k] = (float)slot[k];
-    }
-}
-
-template <int KPL, int TT, int W>
-hipError_t launch_q(const OmpArgs& a, hipStream_t st) {
-    const size_t lds = a.sel ? (size_t)((a.K + 15) & ~15) : 0;
-    if (a.eps2)
-        hipLaunchKernelGGL((omp_qform_kernel<KPL, TT, W, true>), dim3((unsigned)a.N), dim3(64 * W), lds, st, a);
-    else
-        hipLaunchKernelGGL((omp_qform_kernel<KPL, TT, W, false>), dim3((unsigned)a.N), dim3(64 * W), lds, st, a);
-    return hipGetLastError();
-}
-
-template <int KPL, int TT>
-hipError_t launch_qw(const OmpArgs& a, int W, hipStream_t st) {
-    switch (W) {
-        case 1: return launch_q<KPL, TT, 1>(a, st);
-        case 2: return launch_q<KPL, TT, 2>(a, st);
-        case 4: return launch_q<KPL, TT, 4>(a, st);
-        case 8: return launch_q<KPL, TT, 8>(a, st);
-    }
-    return hipErrorInvalidValue;
-}
-
-}  // namespace
+hipError_t launch_qform_t8(const OmpArgs& a, const QformShape& sh, hipStream_t st);
+hipError_t launch_qform_t16(const OmpArgs& a, const QformShape& sh, hipStream_t st);
+hipError_t launch_qform_t32(const OmpArgs& a, const QformShape& sh, hipStream_t st);
+hipError_t launch_qform_t64(const OmpArgs& a, const QformShape& sh, hipStream_t st);
 
 QformShape qform_shape(int K, int T) {
     QformShape sh{0, 0, 0};
@@ -356,25 +32,11 @@ hipError_t launch_batch_omp_qform(const OmpArgs& a, hipStream_t st) {
     if (a.N <= 0) return hipSuccess;
     const QformShape sh = qform_shape(a.K, a.T);
     if (!sh.kpl) return hipErrorInvalidValue;
-    const int W = sh.waves;
     switch (sh.tt) {
-        case 8:
-            if (sh.kpl == 1) return launch_q<1, 8, 1>(a, st);
-            if (sh.kpl == 2) return launch_q<2, 8, 1>(a, st);
-            if (sh.kpl == 4) return launch_q<4, 8, 1>(a, st);
-            return launch_qw<8, 8>(a, W, st);
-        case 16:
-            if (sh.kpl == 1) return launch_q<1, 16, 1>(a, st);
-            if (sh.kpl == 2) return launch_q<2, 16, 1>(a, st);
-            if (sh.kpl == 4) return launch_q<4, 16, 1>(a, st);
-            return launch_qw<8, 16>(a, W, st);
-        case 32:
-            if (sh.kpl == 1) return launch_q<1, 32, 1>(a, st);
-            if (sh.kpl == 2) return launch_q<2, 32, 1>(a, st);
-            return launch_qw<4, 32>(a, W, st);
-        case 64:
-            if (sh.kpl == 1) return launch_q<1, 64, 1>(a, st);
-            return launch_qw<2, 64>(a, W, st);
+        case 8: return launch_qform_t8(a, sh, st);
+        case 16: return launch_qform_t16(a, sh, st);
+        case 32: return launch_qform_t32(a, sh, st);
+        case 64: return launch_qform_t64(a, sh, st);
     }
     return hipErrorInvalidValue;
 }
