@@ -76,6 +76,19 @@ __device__ __forceinline__ int wave_max_i32_dpp(int v) {
     return max(max(r0, r1), max(r2, r3));
 }
 
+// The same wave max with the row maxima combined by DPP row_bcast:15 (rows
+// 1 and 3 take the last lane of rows 0 and 2) and row_bcast:31 (row 3 takes
+// lane 31): lane 63 ends with the wave max, read with one v_readlane.
+__device__ __forceinline__ int wave_max_i32_bcast(int v) {
+    v = max(v, dpp_i<0xB1>(v));
+    v = max(v, dpp_i<0x4E>(v));
+    v = max(v, dpp_i<0x141>(v));
+    v = max(v, dpp_i<0x140>(v));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Wave sum with DPP row operations (quad swaps, half-row and row mirrors:
 // every lane of a row ends with the same row sum, the adds being commutative
 // pairs), then the four row sums in fixed order.  Deterministic; uniform

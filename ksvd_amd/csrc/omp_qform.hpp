@@ -18,12 +18,16 @@
 // k = (v*64 + l)*KPL + m, m < KPL, contiguous per lane (vector loads of the
 // DtX row and of G columns), with alpha[m] and Q[j][m] (j < TT-1) in VGPRs:
 // TT*KPL <= 128 registers, the per-signal working set T*K*4 bytes spread over
-// the workgroup's register files instead of being re-streamed from L2.
-// Per step, one barrier: each wave's arg-max (DPP max + ballot) and its
-// winning lane's row of Q go to LDS (double-buffered by step parity) ->
-// barrier -> every wave reduces the W candidates (same data, same order:
-// identical decisions, so every wave leaves the loop at the same step), loads
-// its slice of G(:, k_i), reads w from the winner's row and forms q_i.
+// the workgroup's register files (and, for the largest basis, LDS) instead
+// of being re-streamed from L2.  All steps are unrolled, so every index into
+// Q is a compile-time constant.
+// Per step, one barrier: each wave's arg-max (DPP max + ballot) goes to LDS
+// as one 16-byte record (alpha^2 bits, atom, alpha) with its winning lane's
+// row of Q (double-buffered by step parity) -> barrier -> lanes w < W of
+// every wave read the W records and reduce them by DPP (same data, same
+// order: identical decisions, so every wave leaves the loop at the same
+// step), load their slice of G(:, k_i), read w from the winner's row and
+// form q_i.
 //
 // The pivot d = 1 - ||w||^2 is recomputed in FP64 from G(I, k_i) and L when
 // it falls below pivot_fp64, as in the wave kernels.
@@ -37,8 +41,9 @@ namespace qform_detail {
 // out[m] = p[kb + m] for kb + m < K, else 0.  `vec`: 16-byte (or KPL*4-byte)
 // aligned vector loads are legal for this row (K a multiple of min(KPL, 4)).
 template <int KPL>
-__device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, int K, bool vec, float* out) {
-    if (vec && kb + KPL <= K) {
+__device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, int K, bool vec, bool full,
+                                           float* out) {
+    if (full || (vec && kb + KPL <= K)) {
         if constexpr (KPL == 1) {
             out[0] = p[kb];
         } else if constexpr (KPL == 2) {
@@ -113,8 +118,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
     __shared__ float ys[TT];       // y = L^-1 alpha0_I
     // per step parity: each wave's candidate (alpha^2, atom, signed alpha) and
     // the candidate's row of Q (zero beyond the step), which is w if it wins
-    __shared__ int cand_v[2][W], cand_k[2][W];
-    __shared__ float cand_a[2][W];
+    __shared__ int4 cand[2][W];
     __shared__ __align__(16) float wrow[2][W][TT];  // rows < QR used
     __shared__ __align__(16) float wfix[W][TT];  // per wave: w recomputed in FP64 (zero beyond the step)
     // columns QR .. QT-1 of Q: column j of wave v at Qs[v][j - QR][lane*KPL + m]
@@ -130,6 +134,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
     const int kb = (v * 64 + lane) * KPL;
     const float* __restrict__ G = a.G;
     const bool vec = (K % (KPL < 4 ? KPL : 4)) == 0;
+    const bool full = vec && 64 * W * KPL <= K;
 
     // alpha of this lane's atoms (0 for padding atoms k >= K: they sit above
     // every real atom, so with the lowest-index tie rule they never win
@@ -194,16 +199,14 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
                 const int c2 = __float_as_int(al[m] * al[m]);
                 if (c2 > bv) { bv = c2; bm = m; }
             }
-            const int M = wave_max_i32_dpp(bv);
+            const int M = wave_max_i32_bcast(bv);
             const int wl = __builtin_ctzll(__ballot(bv == M));
             if (lane == wl) {
                 float av = al[0];
 #pragma unroll
                 for (int m = 1; m < KPL; ++m)
                     if (m == bm) av = al[m];
-                cand_v[b][v] = M;
-                cand_k[b][v] = kb + bm;
-                cand_a[b][v] = av;
+                cand[b][v] = make_int4(M, kb + bm, __float_as_int(av), 0);
                 // rows j < min(i, QR) of Q for atom kb + bm (groups of UG; rows >= i
                 // are zero); rows QR.. are read from Qs by the other waves
 #pragma unroll
@@ -224,24 +227,29 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             }
             __syncthreads();
 
-            // the block's arg-max over the W wave winners: strict '>' in wave
-            // order keeps the lowest atom among equal values
-            int best = cand_v[b][0];
-            int pos = cand_k[b][0], vo = 0;
-#pragma unroll
-            for (int w2 = 1; w2 < W; ++w2) {
-                const int cv = cand_v[b][w2];
-                if (cv > best) { best = cv; pos = cand_k[b][w2]; vo = w2; }
+            // the block's arg-max over the W records, lanes w < W holding
+            // record w: max by DPP, then the lowest wave holding it (the lowest
+            // atom among equal values)
+            int cv = INT_MIN, ck = 0, ca = 0;
+            if (lane < W) {
+                const int4 r = cand[b][lane & (W - 1)];
+                cv = r.x; ck = r.y; ca = r.z;
             }
-            pos = uniform(pos);
-            vo = uniform(vo);
-            const float bestf = __int_as_float(uniform(best));
+            int mx = cv;
+            if constexpr (W >= 2) mx = max(mx, dpp_i<0xB1>(mx));
+            if constexpr (W >= 4) mx = max(mx, dpp_i<0x4E>(mx));
+            if constexpr (W >= 8) mx = max(mx, dpp_i<0x141>(mx));
+            const int best = __builtin_amdgcn_readfirstlane(mx);
+            const int vo = (int)__builtin_ctzll(__ballot(lane < W && cv == best) | (1ull << 63)) & 63;
+            const int pos = __builtin_amdgcn_readlane(ck, vo);
+            const float apos = __int_as_float(__builtin_amdgcn_readlane(ca, vo));
+            const float bestf = __int_as_float(best);
             // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
             // pos >= K: only padding atoms left (every real alpha NaN or 0)
             if ((unsigned)pos >= (unsigned)K || !((double)bestf >= 1e-14)) break;
             if (__ballot(lane < i && sp.ind == pos) != 0ull) break;
             float g[KPL];
-            load_slice<KPL>(G + (long)pos * K, kb, K, vec, g);
+            load_slice<KPL>(G + (long)pos * K, kb, K, vec, full, g);
             // w_j = q_j(pos), one per lane j < i; d = 1 - ||w||^2 from a
             // lane-parallel sum
             const float* wsrc = wrow[b][vo];
@@ -290,7 +298,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
                 rl = __builtin_amdgcn_rsqf(d);
                 l = d * rl;
             }
-            const float y = cand_a[b][vo] * rl;
+            const float y = apos * rl;
             // w_j for the LDS-held rows, now (scalar registers), so that w_lane is
             // dead during the Q w pass
             float wq[QL > 0 ? QL : 1];
@@ -318,7 +326,6 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
 #pragma unroll
                             for (int m = 0; m < KPL; ++m) acc[m] = fmaf(Q[j + e][m], wj[e], acc[m]);
                         }
-                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
             // rows QR <= j < i from LDS
