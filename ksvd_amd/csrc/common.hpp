@@ -84,9 +84,27 @@ __device__ __forceinline__ int wave_max_i32_bcast(int v) {
     v = max(v, dpp_i<0x4E>(v));
     v = max(v, dpp_i<0x141>(v));
     v = max(v, dpp_i<0x140>(v));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+    // INT_MIN (the identity of max) in the rows left out lets the compiler
+    // fold each step into one v_max_i32_dpp
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x143, 0xC, 0xF, false));
     return __builtin_amdgcn_readlane(v, 63);
+}
+
+// lanes_sum_dpp with the row sums combined by row_bcast:15 / row_bcast:31
+// (the same additions as lanes_sum_dpp, one v_readlane; -0.0, the identity
+// of the addition, in the rows left out)
+template <int NR>
+__device__ __forceinline__ float lanes_sum_bcast(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    v += dpp_f<0x141>(v);
+    v += dpp_f<0x140>(v);
+    if constexpr (NR == 1) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    v += __int_as_float(__builtin_amdgcn_update_dpp((int)0x80000000, __float_as_int(v), 0x142, 0xA, 0xF, false));
+    if constexpr (NR == 2) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    v += __int_as_float(__builtin_amdgcn_update_dpp((int)0x80000000, __float_as_int(v), 0x143, 0xC, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // Wave sum with DPP row operations (quad swaps, half-row and row mirrors:

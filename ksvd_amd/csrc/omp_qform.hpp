@@ -43,7 +43,22 @@ namespace qform_detail {
 template <int KPL>
 __device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, int K, bool vec, bool full,
                                            float* out) {
-    if (full || (vec && kb + KPL <= K)) {
+    if (full) {
+        if constexpr (KPL == 1) {
+            out[0] = p[kb];
+        } else if constexpr (KPL == 2) {
+            const float2 u = *reinterpret_cast<const float2*>(p + kb);
+            out[0] = u.x; out[1] = u.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < KPL / 4; ++q) {
+                const float4 u = *reinterpret_cast<const float4*>(p + kb + 4 * q);
+                out[4 * q] = u.x; out[4 * q + 1] = u.y; out[4 * q + 2] = u.z; out[4 * q + 3] = u.w;
+            }
+        }
+        return;
+    }
+    if (vec && kb + KPL <= K) {
         if constexpr (KPL == 1) {
             out[0] = p[kb];
         } else if constexpr (KPL == 2) {
@@ -193,15 +208,20 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             // this wave's arg-max of alpha^2: largest value, lowest atom on ties
             // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges).
             // alpha^2 >= 0, so its bit pattern orders like a signed int.
-            int bv = -1, bm = 0;
+            int c2[KPL];
 #pragma unroll
-            for (int m = 0; m < KPL; ++m) {
-                const int c2 = __float_as_int(al[m] * al[m]);
-                if (c2 > bv) { bv = c2; bm = m; }
-            }
+            for (int m = 0; m < KPL; ++m) c2[m] = __float_as_int(al[m] * al[m]);
+            int bv = -1;
+#pragma unroll
+            for (int m = 0; m < KPL; ++m) bv = max(bv, c2[m]);
             const int M = wave_max_i32_bcast(bv);
-            const int wl = __builtin_ctzll(__ballot(bv == M));
+            const int wl = __builtin_ctzll(__builtin_amdgcn_ballot_w64(bv == M));
             if (lane == wl) {
+                // lowest slot holding the lane's maximum
+                int bm = KPL - 1;
+#pragma unroll
+                for (int m = KPL - 2; m >= 0; --m)
+                    if (c2[m] == bv) bm = m;
                 float av = al[0];
 #pragma unroll
                 for (int m = 1; m < KPL; ++m)
@@ -240,14 +260,16 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             if constexpr (W >= 4) mx = max(mx, dpp_i<0x4E>(mx));
             if constexpr (W >= 8) mx = max(mx, dpp_i<0x141>(mx));
             const int best = __builtin_amdgcn_readfirstlane(mx);
-            const int vo = (int)__builtin_ctzll(__ballot(lane < W && cv == best) | (1ull << 63)) & 63;
+            const int vo = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(lane < W && cv == best) | (1ull << 63)) & 63;
             const int pos = __builtin_amdgcn_readlane(ck, vo);
             const float apos = __int_as_float(__builtin_amdgcn_readlane(ca, vo));
-            const float bestf = __int_as_float(best);
             // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
             // pos >= K: only padding atoms left (every real alpha NaN or 0)
-            if ((unsigned)pos >= (unsigned)K || !((double)bestf >= 1e-14)) break;
-            if (__ballot(lane < i && sp.ind == pos) != 0ull) break;
+            // ((double)alpha^2 >= 1e-14 holds exactly for the bit patterns from
+            // 0x283424DD up to +inf)
+            if ((unsigned)pos >= (unsigned)K || (unsigned)(best - 0x283424DD) > (unsigned)(0x7F800000 - 0x283424DD))
+                break;
+            if (__builtin_amdgcn_ballot_w64(lane < i && sp.ind == pos) != 0ull) break;
             float g[KPL];
             load_slice<KPL>(G + (long)pos * K, kb, K, vec, full, g);
             // w_j = q_j(pos), one per lane j < i; d = 1 - ||w||^2 from a
@@ -262,9 +284,9 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
                     if (lane >= QR) wp = &Qs[vo][lane - QR][pos & (64 * KPL - 1)];
                 wv = *wp;
             }
-            const float d = 1.f - lanes_sum_dpp<(TT + 15) / 16>(wv * wv);
+            const float d = 1.f - lanes_sum_bcast<(TT + 15) / 16>(wv * wv);
             double dd = (double)d;
-            const bool redo = i > 0 && dd < (double)a.pivot_fp64;
+            const bool redo = i > 0 && d < a.pivot_fp64;  // both FP32: exact
             if (redo) {
                 // ill-conditioned: w = L^-1 G(I, pos) and d again in FP64
                 // (backsubt_indexed_append_L / sumSquareValues, S/OMP.cu:169-182, 221-228)
