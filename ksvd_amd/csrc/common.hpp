@@ -257,6 +257,7 @@ struct DuWork {
     int32_t* chunk_hist; // K x nchunks
     int32_t* list_pos;   // N*T
     float* list_g;       // N*T: gamma of each list entry at list-build time
+    float* list_nx;      // N*T: gamma of the entry's signal on the next atom in the sweep order (0 if unused)
     float* R;            // n x N restricted residuals X - D Gamma
     float* apart;        // kDuBlocks x (n rounded up to 4) partial a
     float* bpart;        // kDuBlocks x 2 partial arg-max (special case)

@@ -118,16 +118,33 @@ __global__ __launch_bounds__(1024) void du_scan_ptr_kernel(const int32_t* __rest
 
 // One wave per chunk: signals in ascending order, lane j = slot j; the atoms of
 // one signal are distinct, so the LDS cursors never collide within a step.
+// With `list_nx` (the pipelined sweep) each entry (signal s, atom p) also
+// records, against the sweep order `perm`, whether s uses the atom updated
+// just before p (bit 31 of list_pos: its phase-B pass already covers s) and
+// the coefficient of s on the atom updated just after p (list_nx, 0 when s
+// does not use it).
+constexpr int32_t kPrevFlag = (int32_t)0x80000000;
+__device__ __forceinline__ int32_t entry_pos(int32_t e) { return e & 0x7FFFFFFF; }
+
 __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__ idx, const int32_t* __restrict__ cnt,
                                                      int N, int T, int K, int CH,
                                                      const int32_t* __restrict__ chunk_off,
                                                      const int32_t* __restrict__ ptr,
                                                      int32_t* __restrict__ list_pos,
-                                                     const float* __restrict__ val, float* __restrict__ list_g) {
+                                                     const float* __restrict__ val, float* __restrict__ list_g,
+                                                     const uint32_t* __restrict__ perm, float* __restrict__ list_nx) {
     extern __shared__ int32_t cur[];
+    int32_t* rk = cur + K;  // sweep rank of each atom
+    int32_t* at = rk + K;   // per signal: slot + 1 of its atom at each rank, 0 elsewhere
     const int lane = threadIdx.x;
     const long c = blockIdx.x;
     for (int k = lane; k < K; k += 64) cur[k] = ptr[k] + chunk_off[c * K + k];
+    if (list_nx) {
+        for (int k = lane; k < K; k += 64) {
+            rk[perm[k]] = k;
+            at[k] = 0;
+        }
+    }
     wave_lds_fence();
     const long s0 = c * CH;
     const long s1 = min((long)N, s0 + CH);
@@ -140,7 +157,24 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
             nc = cnt[s + 1];
             nk = (lane < nc) ? idx[(s + 1) * T + lane] : 0;
         }
-        if (lane < cc) {
+        if (list_nx) {
+            const float vl = (lane < cc) ? val[s * T + lane] : 0.f;
+            const int r = (lane < cc) ? rk[kk] : 0;
+            if (lane < cc) at[r] = lane + 1;
+            wave_lds_fence();
+            const int pv = (lane < cc && r > 0) ? at[r - 1] : 0;
+            const int nx = (lane < cc && r + 1 < K) ? at[r + 1] : 0;
+            const float gx = __shfl(vl, nx > 0 ? nx - 1 : 0);
+            wave_lds_fence();
+            if (lane < cc) {
+                at[r] = 0;
+                const int p = cur[kk];
+                cur[kk] = p + 1;
+                list_pos[p] = (int32_t)(s * T + lane) | (pv ? kPrevFlag : 0);
+                list_g[p] = vl;
+                list_nx[p] = nx ? gx : 0.f;
+            }
+        } else if (lane < cc) {
             const int p = cur[kk];
             cur[kk] = p + 1;
             list_pos[p] = (int32_t)(s * T + lane);
@@ -470,6 +504,73 @@ __device__ __forceinline__ void du_commit_column(int jj, const uint32_t* __restr
     }
 }
 
+// The unused-atom special case (S/DictionaryUpdate.cu:238-302, SCupdate
+// :406-427) for atom p, run by every block of the launch: errors ||r_b||^2 of
+// the first min(N - used, M) live unused-list entries, arg-max (lowest index
+// on ties) through block partials and a last-block ticket, then
+// d_p = x / ||x||, swap-remove, replaced, bitmap.
+__device__ void du_special_case(int p, const float* __restrict__ X, float* __restrict__ D,
+                                const float* __restrict__ R, int32_t* __restrict__ unused,
+                                int32_t* __restrict__ usc, int32_t* __restrict__ bitmap,
+                                int32_t* __restrict__ replaced, int n, int N, int M, float* __restrict__ bpart,
+                                int32_t* __restrict__ ticket, float* rv, int* ri, int* flag_sh, float* wred) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int G = gridDim.x;
+    const int counter = *usc;
+    const int lim = min(M, N - counter);
+    float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
+    int bi = 0;
+    for (int b = blockIdx.x * DU_WAVES + w; b < lim; b += G * DU_WAVES) {
+        const long sig = unused[b];
+        float e = 0.f;
+        for (int t = lane; t < n; t += 64) {
+            const float r = R[sig * n + t];
+            e += r * r;
+        }
+        e = wave_sum_dpp(e);
+        if (e > bv) { bv = e; bi = b; }  // b ascending per wave: keeps the lowest on ties
+    }
+    if (lane == 0) { rv[w] = bv; ri[w] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = rv[0];
+        int i = ri[0];
+        for (int q = 1; q < DU_WAVES; ++q)
+            if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
+        bpart[2 * blockIdx.x] = v;
+        bpart[2 * blockIdx.x + 1] = __int_as_float(i);
+    }
+    if (!last_block(ticket, flag_sh)) return;
+    if (threadIdx.x == 0) {
+        float v = bpart[0];
+        int i = __float_as_int(bpart[1]);
+        for (int q = 1; q < G; ++q) {
+            const float vq = bpart[2 * q];
+            const int iq = __float_as_int(bpart[2 * q + 1]);
+            if (vq > v || (vq == v && iq < i)) { v = vq; i = iq; }
+        }
+        ri[0] = i;
+    }
+    __syncthreads();
+    const int best = ri[0];
+    const long sig = unused[best];
+    float sq = 0.f;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const float x = X[sig * n + t];
+        sq += x * x;
+    }
+    sq = block_sum(sq, wred);
+    const float nrm = sqrtf(sq);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unused[best] = unused[N - counter - 1];
+        *usc = counter + 1;
+        replaced[p] = 1;
+        bitmap[sig] = 0;
+    }
+}
+
 template <int NCH>
 __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
@@ -493,60 +594,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     const bool vec = (n & 3) == 0;
     if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, abuf, anorm, n);
     if (Np == 0) {
-        // unused-atom special case (S/DictionaryUpdate.cu:238-302, SCupdate :406-427)
-        const int counter = *usc;
-        const int lim = min(M, N - counter);
-        float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
-        int bi = 0;
-        for (int b = blockIdx.x * DU_WAVES + w; b < lim; b += G * DU_WAVES) {
-            const long sig = unused[b];
-            float e = 0.f;
-            for (int t = lane; t < n; t += 64) {
-                const float r = R[sig * n + t];
-                e += r * r;
-            }
-            e = wave_sum_dpp(e);
-            if (e > bv) { bv = e; bi = b; }  // b ascending per wave: keeps the lowest on ties
-        }
-        if (lane == 0) { rv[w] = bv; ri[w] = bi; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float v = rv[0];
-            int i = ri[0];
-            for (int q = 1; q < DU_WAVES; ++q)
-                if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
-            bpart[2 * blockIdx.x] = v;
-            bpart[2 * blockIdx.x + 1] = __int_as_float(i);
-        }
-        if (!last_block(ticket, &flag_sh)) return;
-        if (threadIdx.x == 0) {
-            float v = bpart[0];
-            int i = __float_as_int(bpart[1]);
-            for (int q = 1; q < G; ++q) {
-                const float vq = bpart[2 * q];
-                const int iq = __float_as_int(bpart[2 * q + 1]);
-                if (vq > v || (vq == v && iq < i)) { v = vq; i = iq; }
-            }
-            ri[0] = i;
-        }
-        __syncthreads();
-        const int best = ri[0];
-        const long sig = unused[best];
-        float sq = 0.f;
-        for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            const float x = X[sig * n + t];
-            sq += x * x;
-        }
-        sq = block_sum(sq, wred);
-        const float nrm = sqrtf(sq);
-        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unused[best] = unused[N - counter - 1];
-            *usc = counter + 1;
-            replaced[p] = 1;
-            bitmap[sig] = 0;
-        }
+        du_special_case(p, X, D, R, unused, usc, bitmap, replaced, n, N, M, bpart, ticket, rv, ri, &flag_sh, wred);
         return;
     }
     // a = sum_{s in I_p} gamma_s (r_s + d_p gamma_s)
@@ -561,7 +609,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     }
     for (int base = gw; base < Np; base += 64 * GW) {
         const int mk = base + lane * GW;
-        const int pos = (mk < Np) ? list_pos[beg + mk] : 0;
+        const int pos = (mk < Np) ? entry_pos(list_pos[beg + mk]) : 0;
         const float gl = (mk < Np) ? list_g[beg + mk] : 0.f;
         const int sl = pos / T;
         const int nk = min(64, (Np - base + GW - 1) / GW);
@@ -655,76 +703,141 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const uint32_t
     if (threadIdx.x == 0) anorm[(jj & 1) * DU_RB + r] = sq;
 }
 
+// Phase B of the atom at sweep position jj fused with phase A of the atom at
+// jj + 1 (the pipelined sweep).  Phase B (S/Multiplication.cu:198-225): for
+// s in I_p, e = r_s + d_p gamma_s, gamma_new = d_new^T e, r_s <- e - d_new
+// gamma_new, with d_new = a / ||a|| from du_a_reduce_kernel.  A signal the
+// next atom q also uses contributes gamma_q (r_s + d_q gamma_q) to q's a from
+// its updated residual, right here (list_nx holds gamma_q); every other
+// signal of I_q is unaffected by this update and contributes from R.  Block
+// partials of q's a go to apart (reduced by du_a_reduce_kernel(jj + 1)).
+// An unused atom p (N_p = 0, jj > 0; position 0 is handled by du_a_kernel)
+// takes the special case here, after q's partials are written.
 template <int NCH>
-__global__ __launch_bounds__(512) void du_b_kernel(int jj, const uint32_t* __restrict__ perm,
-                                                   const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
-                                                   const int32_t* __restrict__ list_pos,
-                                                   const float* __restrict__ list_g, float* __restrict__ val,
-                                                   const float* __restrict__ D, float* __restrict__ R,
-                                                   const float* __restrict__ abuf, const float* __restrict__ anorm,
-                                                   int n, int T) {
+__global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_t* __restrict__ perm,
+                                                    const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ list_pos,
+                                                    const float* __restrict__ list_g, const float* __restrict__ list_nx,
+                                                    const float* __restrict__ X, float* __restrict__ val,
+                                                    float* __restrict__ D, float* __restrict__ R,
+                                                    int32_t* __restrict__ unused, int32_t* __restrict__ usc,
+                                                    int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
+                                                    int N, int T, int M, float* __restrict__ apart,
+                                                    float* __restrict__ bpart, const float* __restrict__ abuf,
+                                                    const float* __restrict__ anorm, int32_t* __restrict__ ticket) {
+    __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a of the next atom
+    __shared__ float rv[DU_WAVES];
+    __shared__ int ri[DU_WAVES];
+    __shared__ int flag_sh;
+    __shared__ float wred[16];
     const int p = (int)perm[jj];
     const int Np = count[p];
-    if (Np == 0) return;
+    const int q = (jj + 1 < K) ? (int)perm[jj + 1] : -1;
+    const int Nq = (q >= 0) ? count[q] : 0;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int G = gridDim.x;
     const bool vec = (n & 3) == 0;
-    const int beg = ptr[p];
-    const int gw = blockIdx.x * DU_WAVES + w, GW = gridDim.x * DU_WAVES;
-    const float* dp = D + (long)p * n;
-    const float* aq = abuf + (long)(jj & 1) * n;
-    const float nrm = du_anorm(jj, anorm);
+    const int gw = blockIdx.x * DU_WAVES + w, GW = G * DU_WAVES;
+    if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, abuf, anorm, n);
+    float4 dq[NCH], acc[NCH];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        dq[c] = (Nq > 0) ? ld4(D + (long)q * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+        acc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // per wave, in step: a batch of up to FB phase-B entries of p and up to FB
+    // phase-A entries of q, all rows loaded before either is used
+    constexpr int FB = FQ / NCH > 0 ? FQ / NCH : 1;
+    const float nrm = (Np > 0) ? du_anorm(jj, anorm) : 1.f;
     float4 dv[NCH], dn[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
-        dv[c] = ld4(dp, 256 * c + 4 * lane, n, vec);
-        const float4 a4 = ld4(aq, 256 * c + 4 * lane, n, vec);
+        dv[c] = (Np > 0) ? ld4(D + (long)p * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 a4 =
+            (Np > 0) ? ld4(abuf + (long)(jj & 1) * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
         dn[c] = make_float4(a4.x / nrm, a4.y / nrm, a4.z / nrm, a4.w / nrm);  // d_new = a / ||a||
     }
-    for (int base = gw; base < Np; base += 64 * GW) {
+    const int bp = (Np > 0) ? ptr[p] : 0, bq = (Nq > 0) ? ptr[q] : 0;
+    const int Nmax = max(Np, Nq);
+    for (int base = gw; base < Nmax; base += 64 * GW) {
         const int mk = base + lane * GW;
-        const int pos = (mk < Np) ? list_pos[beg + mk] : 0;
-        const float gl = (mk < Np) ? list_g[beg + mk] : 0.f;
-        const int sl = pos / T;
-        const int nk = min(64, (Np - base + GW - 1) / GW);
-        for (int k = 0; k < nk; k += FQ) {
-            float4 e[FQ][NCH];
-            float dot[FQ];
+        const int posb = (mk < Np) ? entry_pos(list_pos[bp + mk]) : 0;
+        const float gb = (mk < Np) ? list_g[bp + mk] : 0.f;
+        const float xb = (mk < Np) ? list_nx[bp + mk] : 0.f;
+        const int slb = posb / T;
+        const int nkb = (base < Np) ? min(64, (Np - base + GW - 1) / GW) : 0;
+        const int32_t ent = (mk < Nq) ? list_pos[bq + mk] : kPrevFlag;
+        const int sla = entry_pos(ent) / T;
+        const int sha = ent < 0;  // shared with p: added by the phase-B part
+        const float ga = (mk < Nq) ? list_g[bq + mk] : 0.f;
+        const int nka = (base < Nq) ? min(64, (Nq - base + GW - 1) / GW) : 0;
+        for (int k = 0; k < max(nkb, nka); k += FB) {
+            float4 e[FB][NCH], r[FB][NCH];
+            float g2[FB];
 #pragma unroll
-            for (int u = 0; u < FQ; ++u) {
-                const bool ok = k + u < nk;
-                const long s = readlane(sl, ok ? k + u : 0);
-                const float g = ok ? readlane(gl, k + u) : 0.f;
-                dot[u] = 0.f;
+            for (int u = 0; u < FB; ++u) {
+                const bool okb = k + u < nkb;
+                const long s = readlane(slb, okb ? k + u : 0);
+                const float g = okb ? readlane(gb, k + u) : 0.f;
 #pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    // e = r_s + d_p gamma_s
-                    e[u][c] = ok ? f4fma(g, dv[c], ld4(R + s * n, 256 * c + 4 * lane, n, vec))
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int c = 0; c < NCH; ++c)  // e = r_s + d_p gamma_s
+                    e[u][c] = okb ? f4fma(g, dv[c], ld4(R + s * n, 256 * c + 4 * lane, n, vec))
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+                const bool oka = k + u < nka && !readlane(sha, k + u < nka ? k + u : 0);
+                const long sa = readlane(sla, oka ? k + u : 0);
+                g2[u] = oka ? readlane(ga, k + u) : 0.f;
+#pragma unroll
+                for (int c = 0; c < NCH; ++c)
+                    r[u][c] = oka ? ld4(R + sa * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            // phase B: gamma_new = d_new^T e, r_s <- e - d_new gamma_new, and
+            // gamma_q (r_s + d_q gamma_q) for the signals q also uses
+#pragma unroll
+            for (int u = 0; u < FB; ++u) {
+                if (k + u < nkb) {
+                    float dot = 0.f;
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) {
+                        dot = fmaf(dn[c].x, e[u][c].x, dot);
+                        dot = fmaf(dn[c].y, e[u][c].y, dot);
+                        dot = fmaf(dn[c].z, e[u][c].z, dot);
+                        dot = fmaf(dn[c].w, e[u][c].w, dot);
+                    }
+                    dot = wave_sum_dpp(dot);
+                    const long s = readlane(slb, k + u);
+                    const float gq = readlane(xb, k + u);
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) {
+                        const float4 rn = f4fma(-dot, dn[c], e[u][c]);
+                        st4(R + s * n, 256 * c + 4 * lane, n, vec, rn);
+                        if (gq != 0.f) acc[c] = f4fma(gq, f4fma(gq, dq[c], rn), acc[c]);
+                    }
+                    if (lane == 0) val[readlane(posb, k + u)] = dot;
                 }
             }
+            // phase A of q over its signals not shared with p
 #pragma unroll
-            for (int u = 0; u < FQ; ++u) {
+            for (int u = 0; u < FB; ++u)
 #pragma unroll
-                for (int c = 0; c < NCH; ++c) {
-                    dot[u] = fmaf(dn[c].x, e[u][c].x, dot[u]);
-                    dot[u] = fmaf(dn[c].y, e[u][c].y, dot[u]);
-                    dot[u] = fmaf(dn[c].z, e[u][c].z, dot[u]);
-                    dot[u] = fmaf(dn[c].w, e[u][c].w, dot[u]);
-                }
-                dot[u] = wave_sum_dpp(dot[u]);  // gamma_new = d_new^T (r_s + d_p gamma_s)
-            }
-#pragma unroll
-            for (int u = 0; u < FQ; ++u) {
-                if (k + u < nk) {
-                    const long s = readlane(sl, k + u);
-#pragma unroll
-                    for (int c = 0; c < NCH; ++c)
-                        st4(R + s * n, 256 * c + 4 * lane, n, vec, f4fma(-dot[u], dn[c], e[u][c]));
-                    if (lane == 0) val[readlane(pos, k + u)] = dot[u];
-                }
-            }
+                for (int c = 0; c < NCH; ++c) acc[c] = f4fma(g2[u], f4fma(g2[u], dq[c], r[u][c]), acc[c]);
         }
     }
+    if (Nq > 0) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) part[w][64 * c + lane] = acc[c];
+        __syncthreads();
+        // block partial, rows padded to np4 = n rounded up to 4 floats
+        const int np4 = (n + 3) & ~3;
+        const float* pf = reinterpret_cast<const float*>(&part[0][0]);
+        for (int t = threadIdx.x; t < np4; t += blockDim.x) {
+            float sacc = 0.f;
+#pragma unroll
+            for (int qq = 0; qq < DU_WAVES; ++qq) sacc += pf[qq * 256 * NCH + t];
+            apart[(long)blockIdx.x * np4 + t] = sacc;
+        }
+    }
+    if (Np == 0 && jj > 0) du_special_case(p, X, D, R, unused, usc, bitmap, replaced, n, N, M, bpart, ticket, rv, ri,
+                                           &flag_sh, wred);
 }
 
 __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const int32_t* __restrict__ count,
@@ -782,14 +895,21 @@ template <int NCH>
 void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hipStream_t st) {
     float* abuf = w.dnew;                 // 2 x n: a by sweep-position parity
     float* anorm = w.dnew + 2 * w.n;      // 2 x DU_RB: partial sums of a^2
+    // sweep position 0: phase A (or the special case) and its reduction; then
+    // per position jj: phase B of jj fused with phase A of jj + 1, and the
+    // reduction of jj + 1 (two launches per atom)
+    hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
+                       w.list_pos, w.list_g, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
+                       M, w.apart, w.bpart, abuf, anorm, w.tickets);
+    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.perm_dev, w.count, w.apart, abuf, anorm,
+                       w.n, DU_BLOCKS);
     for (int jj = 0; jj < w.K; ++jj) {
-        hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
-                           w.list_pos, w.list_g, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N,
-                           w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets);
-        hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj, w.perm_dev, w.count, w.apart, abuf,
-                           anorm, w.n, DU_BLOCKS);
-        hipLaunchKernelGGL((du_b_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.perm_dev, w.count, w.ptr,
-                           w.list_pos, w.list_g, val, D, w.R, abuf, anorm, w.n, w.T);
+        hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
+                           w.ptr, w.list_pos, w.list_g, w.list_nx, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
+                           w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets);
+        if (jj + 1 < w.K)
+            hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.perm_dev, w.count,
+                               w.apart, abuf, anorm, w.n, DU_BLOCKS);
     }
     hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n);
 }
@@ -1026,8 +1146,11 @@ hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* c
                        w.chunk, w.chunk_hist);
     hipLaunchKernelGGL(du_scan_chunks_kernel, dim3(w.K), dim3(256), 0, st, w.chunk_hist, w.nchunks, w.K, w.count);
     hipLaunchKernelGGL(du_scan_ptr_kernel, dim3(1), dim3(1024), 0, st, w.count, w.K, w.ptr);
-    hipLaunchKernelGGL(du_fill_kernel, dim3(w.nchunks), dim3(64), (size_t)w.K * 4, st, idx, cnt, w.N, w.T, w.K,
-                       w.chunk, w.chunk_hist, w.ptr, w.list_pos, val, val ? w.list_g : nullptr);
+    // the pipelined sweep (n <= 1024) also records each entry's neighbours in the sweep order
+    float* nx = (val && w.n <= 1024) ? w.list_nx : nullptr;
+    hipLaunchKernelGGL(du_fill_kernel, dim3(w.nchunks), dim3(64), (size_t)w.K * 4 * (nx ? 3 : 1), st, idx, cnt, w.N,
+                       w.T, w.K, w.chunk, w.chunk_hist, w.ptr, w.list_pos, val, val ? w.list_g : nullptr, w.perm_dev,
+                       nx);
     return hipGetLastError();
 }
 

@@ -380,7 +380,12 @@ def test_execute_iterations_exact_fit_init(o64):
     say("rmse gpu", rmse, "oracle f64", ref["rmse"], "oracle f32", ref32["rmse"], "f32/f64 spread", spread)
     assert spread > 1e-5  # the input is precision-sensitive, as documented
     np.testing.assert_allclose(rmse[0], ref["rmse"][0], rtol=1e-3)
-    np.testing.assert_allclose(rmse[1:], ref["rmse"][1:], rtol=1e-4)
+    # later iterations: as close to the FP64 oracle as the FP32 port of the
+    # same algorithm is (its own spread on this input, at least 1e-4), and
+    # tight against that FP32 port
+    spread_later = np.abs(ref32["rmse"][1:] / ref["rmse"][1:] - 1)
+    assert np.all(np.abs(np.asarray(rmse[1:]) / ref["rmse"][1:] - 1) <= np.maximum(1e-4, 2 * spread_later))
+    np.testing.assert_allclose(rmse[1:], ref32["rmse"][1:], rtol=2e-5)
     eng.close()
 
 
