@@ -45,9 +45,12 @@ __device__ __forceinline__ void to_bf16(const float4& p, const float4& q, bf16x8
     hi[4] = (__bf16)q.x; hi[5] = (__bf16)q.y; hi[6] = (__bf16)q.z; hi[7] = (__bf16)q.w;
 }
 
-// Loads one 128 x 32 tile (rows of the tile are columns of the matrix).
-__device__ __forceinline__ void load_tile(float* __restrict__ s, const float* __restrict__ M, int rows,
-                                          long col0, long cols, int t0, bool vec) {
+// One 128 x 32 tile (rows of the tile are columns of the matrix): each
+// thread fetches 4 float4s into registers (fetch_tile), later stored to LDS
+// (put_tile), so the next chunk's global loads are in flight while the MFMAs
+// of the current one run.
+__device__ __forceinline__ void fetch_tile(float4* v, const float* __restrict__ M, int rows, long col0, long cols,
+                                           int t0, bool vec) {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -56,19 +59,27 @@ __device__ __forceinline__ void load_tile(float* __restrict__ s, const float* __
         const int c4 = (q & 7) * 4;
         const long col = col0 + row;
         const int t = t0 + c4;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        v[r] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (col < cols) {
             const float* src = M + col * (long)rows + t;
             if (vec && t + 3 < rows) {
-                v = *reinterpret_cast<const float4*>(src);
+                v[r] = *reinterpret_cast<const float4*>(src);
             } else {
-                if (t + 0 < rows) v.x = src[0];
-                if (t + 1 < rows) v.y = src[1];
-                if (t + 2 < rows) v.z = src[2];
-                if (t + 3 < rows) v.w = src[3];
+                if (t + 0 < rows) v[r].x = src[0];
+                if (t + 1 < rows) v[r].y = src[1];
+                if (t + 2 < rows) v[r].z = src[2];
+                if (t + 3 < rows) v[r].w = src[3];
             }
         }
-        *reinterpret_cast<float4*>(s + row * PITCH + c4) = v;
+    }
+}
+
+__device__ __forceinline__ void put_tile(float* __restrict__ s, const float4* v) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int q = tid + 256 * r;
+        *reinterpret_cast<float4*>(s + (q >> 3) * PITCH + (q & 7) * 4) = v[r];
     }
 }
 
@@ -78,11 +89,18 @@ __global__ __launch_bounds__(256) void at_times_b_kernel(const float* __restrict
                                                          float* __restrict__ C, int tilesA) {
     __shared__ float sB[BM * PITCH];
     __shared__ float sA[BN * PITCH];
+    // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+    // XCDs (block b on XCD b % 8), each with its own L2.  Block b works on
+    // XCD x = b % 8 through its r = b / 8-th tile: signal tile
+    // tb = 8 * (r / tilesA) + x and atom tile r % tilesA, so the tilesA
+    // blocks that share a signal tile run on one XCD and read it from HBM
+    // once (a plain b / tilesA order spreads them over all 8 L2s).  The grid
+    // holds tilesB rounded up to a multiple of 8 signal tiles.
     const long blk = blockIdx.x;
-    // Consecutive blocks share the B tile (signals) and walk the A tiles
-    // (atoms), so each signal tile is fetched from HBM once and re-read from L2.
-    const int ta = (int)(blk % tilesA);
-    const long tb = blk / tilesA;
+    const long r8 = blk >> 3;
+    const int ta = (int)(r8 % tilesA);
+    const long tb = 8 * (r8 / tilesA) + (blk & 7);
+    if (tb * BM >= colsB) return;
     const long j0 = tb * BM;
     const int m0 = ta * BN;
     const int lane = threadIdx.x & 63;
@@ -100,10 +118,17 @@ __global__ __launch_bounds__(256) void at_times_b_kernel(const float* __restrict
 
     const int li = lane & 31;
     const int lk = lane >> 5;
+    float4 pb[4], pa[4];
+    fetch_tile(pb, B, rows, j0, colsB, 0, vec);
+    fetch_tile(pa, A, rows, m0, colsA, 0, vec);
     for (int t0 = 0; t0 < rows; t0 += BK) {
-        load_tile(sB, B, rows, j0, colsB, t0, vec);
-        load_tile(sA, A, rows, m0, colsA, t0, vec);
+        put_tile(sB, pb);
+        put_tile(sA, pa);
         __syncthreads();
+        if (t0 + BK < rows) {
+            fetch_tile(pb, B, rows, j0, colsB, t0 + BK, vec);
+            fetch_tile(pa, A, rows, m0, colsA, t0 + BK, vec);
+        }
         if constexpr (FORM == 0) {
 #pragma unroll
             for (int ks = 0; ks < BK; ks += 2) {
@@ -190,7 +215,7 @@ hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B
                              hipStream_t st) {
     if (rows <= 0 || colsA <= 0 || colsB <= 0) return hipSuccess;
     const int tilesA = (colsA + BN - 1) / BN;
-    const long tilesB = (colsB + BM - 1) / BM;
+    const long tilesB = ((colsB + BM - 1) / BM + 7) / 8 * 8;  // whole groups of 8 (one per XCD)
     const long blocks = tilesB * tilesA;
     if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
     dim3 grid((unsigned)blocks), block(256);
