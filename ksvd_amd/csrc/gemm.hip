@@ -17,8 +17,9 @@
 //   BF16   v_mfma_f32_32x32x16_bf16 (2.5 PF)  RNE-rounded BF16 operands
 //   BF16X3 three BF16 MFMAs: hi*hi + hi*lo + lo*hi with x = hi + lo, hi =
 //          bf16(x), lo = bf16(x - hi)  (Henry, Tang & Heinecke, ARITH 2019)
-// Operands are staged through LDS as FP32 and converted to BF16 in registers,
-// so one kernel serves all three forms.
+// Operands are fetched as FP32; the BF16 forms convert them once, on their way
+// into LDS (bf16 hi, and lo for the split form), so one kernel serves all
+// three forms.
 #include "common.hpp"
 
 namespace ksvd {
@@ -29,21 +30,7 @@ constexpr int BM = 128;    // tile extent along colsB (MFMA rows)
 constexpr int BN = 128;    // tile extent along colsA (MFMA cols)
 constexpr int BK = 32;     // reduction chunk
 constexpr int PITCH = 36;  // LDS row pitch in floats (pads 128 B rows: no bank conflicts)
-
-__device__ __forceinline__ void to_bf16_split(const float4& p, const float4& q, bf16x8& hi, bf16x8& lo) {
-    float v[8] = {p.x, p.y, p.z, p.w, q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        __bf16 h = (__bf16)v[e];
-        hi[e] = h;
-        lo[e] = (__bf16)(v[e] - (float)h);
-    }
-}
-
-__device__ __forceinline__ void to_bf16(const float4& p, const float4& q, bf16x8& hi) {
-    hi[0] = (__bf16)p.x; hi[1] = (__bf16)p.y; hi[2] = (__bf16)p.z; hi[3] = (__bf16)p.w;
-    hi[4] = (__bf16)q.x; hi[5] = (__bf16)q.y; hi[6] = (__bf16)q.z; hi[7] = (__bf16)q.w;
-}
+constexpr int PH = 40;     // LDS row pitch in bf16 elements for the BF16 forms (80 B rows)
 
 // One 128 x 32 tile (rows of the tile are columns of the matrix): each
 // thread fetches 4 float4s into registers (fetch_tile), later stored to LDS
@@ -74,6 +61,28 @@ __device__ __forceinline__ void fetch_tile(float4* v, const float* __restrict__ 
     }
 }
 
+// BF16 forms: the tile converted once, at the store to LDS (hi = bf16(x), and
+// for the split form lo = bf16(x - hi)).
+template <int FORM>
+__device__ __forceinline__ void put_tile_bf16(__bf16* __restrict__ sh, __bf16* __restrict__ sl, const float4* v) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int q = tid + 256 * r;
+        const float x[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
+        bf16x4 h, l;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const __bf16 he = (__bf16)x[e];
+            h[e] = he;
+            if constexpr (FORM == 2) l[e] = (__bf16)(x[e] - (float)he);
+        }
+        const int off = (q >> 3) * PH + (q & 7) * 4;
+        *reinterpret_cast<bf16x4*>(sh + off) = h;
+        if constexpr (FORM == 2) *reinterpret_cast<bf16x4*>(sl + off) = l;
+    }
+}
+
 __device__ __forceinline__ void put_tile(float* __restrict__ s, const float4* v) {
     const int tid = threadIdx.x;
 #pragma unroll
@@ -87,8 +96,12 @@ template <int FORM>
 __global__ __launch_bounds__(256) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
                                                          const float* __restrict__ B, long colsB,
                                                          float* __restrict__ C, int tilesA) {
-    __shared__ float sB[BM * PITCH];
-    __shared__ float sA[BN * PITCH];
+    // FP32: tiles in LDS as FP32; BF16 forms: as bf16 hi (and lo)
+    constexpr int NF = FORM == 0 ? BM * PITCH : 1;
+    constexpr int NH = FORM == 0 ? 1 : BM * PH;
+    constexpr int NL = FORM == 2 ? BM * PH : 1;
+    __shared__ float sB[NF], sA[NF];
+    __shared__ __align__(16) __bf16 sBh[NH], sAh[NH], sBl[NL], sAl[NL];
     // XCD-aware tile order: workgroups are dispatched round-robin over the 8
     // XCDs (block b on XCD b % 8), each with its own L2.  Block b works on
     // XCD x = b % 8 through its r = b / 8-th tile: signal tile
@@ -122,8 +135,13 @@ __global__ __launch_bounds__(256) void at_times_b_kernel(const float* __restrict
     fetch_tile(pb, B, rows, j0, colsB, 0, vec);
     fetch_tile(pa, A, rows, m0, colsA, 0, vec);
     for (int t0 = 0; t0 < rows; t0 += BK) {
-        put_tile(sB, pb);
-        put_tile(sA, pa);
+        if constexpr (FORM == 0) {
+            put_tile(sB, pb);
+            put_tile(sA, pa);
+        } else {
+            put_tile_bf16<FORM>(sBh, sBl, pb);
+            put_tile_bf16<FORM>(sAh, sAl, pa);
+        }
         __syncthreads();
         if (t0 + BK < rows) {
             fetch_tile(pb, B, rows, j0, colsB, t0 + BK, vec);
@@ -149,19 +167,15 @@ __global__ __launch_bounds__(256) void at_times_b_kernel(const float* __restrict
                 bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
-                    const float* p = sB + (wi * 64 + a * 32 + li) * PITCH + ks + lk * 8;
-                    float4 p0 = *reinterpret_cast<const float4*>(p);
-                    float4 p1 = *reinterpret_cast<const float4*>(p + 4);
-                    if constexpr (FORM == 2) to_bf16_split(p0, p1, ah[a], al[a]);
-                    else to_bf16(p0, p1, ah[a]);
+                    const int off = (wi * 64 + a * 32 + li) * PH + ks + lk * 8;
+                    ah[a] = *reinterpret_cast<const bf16x8*>(sBh + off);
+                    if constexpr (FORM == 2) al[a] = *reinterpret_cast<const bf16x8*>(sBl + off);
                 }
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    const float* p = sA + (wj * 64 + b * 32 + li) * PITCH + ks + lk * 8;
-                    float4 p0 = *reinterpret_cast<const float4*>(p);
-                    float4 p1 = *reinterpret_cast<const float4*>(p + 4);
-                    if constexpr (FORM == 2) to_bf16_split(p0, p1, bh[b], bl[b]);
-                    else to_bf16(p0, p1, bh[b]);
+                    const int off = (wj * 64 + b * 32 + li) * PH + ks + lk * 8;
+                    bh[b] = *reinterpret_cast<const bf16x8*>(sAh + off);
+                    if constexpr (FORM == 2) bl[b] = *reinterpret_cast<const bf16x8*>(sAl + off);
                 }
 #pragma unroll
                 for (int a = 0; a < 2; ++a)
