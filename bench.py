@@ -4,10 +4,13 @@ Workload (fits one GPU): the precision-form configuration, n=256 signal
 length, K=1024 atoms, N=1,000,000 signals, T=32 atoms per signal, Gaussian
 unit-norm atoms with 25 dB SNR, synthetic and seeded, resident in HBM before
 timing starts.  A step is one Batch-OMP pass over all N signals with the
-headline form: G = D^T D and DtX = D^T X on FP32 MFMA, then the register-basis
-Batch-OMP kernel (one workgroup per signal holding Q = G(:,I) L^-T in VGPRs,
-FP32 throughout).  The wave form that streams G every step (FP32 GEMMs, then
-the FP32-G wave kernel) is timed beside it.
+headline form: G = D^T D and DtX = D^T X as split-BF16x3 MFMA GEMMs (three
+BF16 products per term, FP32 accumulation: FP32-level accuracy), then the
+register-basis Batch-OMP kernel (one workgroup per signal holding
+Q = G(:,I) L^-T on chip, FP32 throughout); --form fp32 runs the GEMMs on FP32
+MFMA instead.  The plain FP32 form that streams G every step (FP32 GEMMs,
+then the FP32-G wave kernel) is timed beside it, and the fraction of signals
+whose supports it matches is reported.
 
     python bench.py [--steps 10] [--warmup 2] [--gpus 1]
 
@@ -133,7 +136,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="fp32", choices=["fp32", "bf16", "bf16x3"], help="Gram GEMM form")
+    ap.add_argument("--form", default="bf16x3", choices=["fp32", "bf16", "bf16x3"],
+                    help="GEMM form of G and DtX (default: split-BF16x3)")
     ap.add_argument("--omp", default="qform", choices=["qform", "wave16", "wave", "mfma"],
                     help="headline Batch-OMP kernel")
     ap.add_argument("--N", type=int, default=1_000_000)
@@ -315,7 +319,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.form == "fp32" else f"f32 (GEMMs {args.form} MFMA, FP32 accumulation)",
         "data": "synthetic",
         "config": {"workload": "C2 precision form: n=256 K=1024 N=1000000 T=32, Gaussian unit atoms, 25 dB SNR",
                    "n": n, "K": K, "N": N, "T": T, "gram_form": args.form, "batch_omp": args.omp,
