@@ -131,13 +131,95 @@ def compare(args):
                                   "rmse_per_iter_gpu": [float(v) for v in rmse]}}), flush=True)
 
 
+def other_config(args):
+    """One JSON line for a secondary configuration, each at its full size:
+    C1 (n=240, K=512, N=1e5, T=16): K-SVD iterations/s over its 10 iterations;
+    C3 (n=128, K=512, N=2e6, per-signal sparsity up to 48, error-constrained
+    stop): Gram + correlation + error-mode Batch-OMP signals/s, with the
+    generating dictionary; C4 (n=64, K=256, B=65536, T=10): Batch-OMP forward
+    + the fixed-support adjoint (gradients w.r.t. D and X) signals/s."""
+    import numpy as np
+    import torch
+
+    from ksvd_amd import Engine, synth
+
+    if args.gpus != 1:
+        print("bench.py runs on one GPU; --gpus must be 1", file=sys.stderr)
+        return 2
+    dev = torch.device("cuda", 0)
+    cfg = args.config
+    p = synth.CONFIGS[cfg]
+    wl = synth.make(cfg, device=dev)
+    X = wl.X.contiguous()
+    n, K, N, T = p["n"], p["K"], p["N"], p["T"]
+    reps = max(1, args.steps)
+    out = {"config": cfg, "workload": dict(n=n, K=K, N=N, T=T), "n_gpus": 1, "data": "synthetic",
+           "higher_is_better": True}
+
+    def timed(fn):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    if cfg == "C1":
+        its = p["iters"]
+        eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant="auto")
+        D0 = wl.D_init.contiguous()
+        rm = []
+
+        def run():
+            rm[:] = eng.ExecuteIterations(X, D0, its)[4]
+        dt = timed(run)
+        out.update(metric="K-SVD iterations/s", value=round(its / dt, 2), unit="iterations/s",
+                   ms_per_iteration=round(dt * 1000.0 / its, 3), iterations=its, gram_form=args.form,
+                   rmse_first_last=[float(rm[0]), float(rm[-1])])
+    elif cfg == "C3":
+        eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant="auto")
+        D = eng.normcols(wl.D_true.contiguous())
+        eps2 = wl.eps2.contiguous()
+        G = eng.gram(D)
+        res = eng.empty_compact()
+
+        def run():
+            eng.gram(D, G)
+            eng.BatchOMP_eps(D, X, G, eps2, out=res)
+        dt = timed(run)
+        cnt = res[2].float()
+        out.update(metric="error-constrained Batch-OMP signals/s", value=round(N / dt, 1), unit="signals/s",
+                   ms_per_pass=round(dt * 1000.0, 3), gram_form=args.form,
+                   mean_support=round(float(cnt.mean()), 3), max_support=int(cnt.max()))
+    else:  # C4
+        eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant="auto")
+        D = eng.normcols(wl.D_init.contiguous())
+        G = eng.gram(D)
+        res = eng.empty_compact()
+        gval = torch.randn((N, T), dtype=torch.float32, device=dev, generator=torch.Generator(device=dev).manual_seed(9))
+        gD, gX = torch.empty_like(D), torch.empty_like(X)
+
+        def run():
+            eng.gram(D, G)
+            idx, val, cnt = eng.BatchOMP(D, X, G, out=res)
+            eng.omp_adjoint(D, X, idx, val, cnt, gval, gD=gD, gX=gX)
+        dt = timed(run)
+        out.update(metric="Batch-OMP forward + adjoint signals/s", value=round(N / dt, 1), unit="signals/s",
+                   ms_per_pass=round(dt * 1000.0, 3), gram_form=args.form)
+    eng.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="bf16x3", choices=["fp32", "bf16", "bf16x3"],
-                    help="GEMM form of G and DtX (default: split-BF16x3)")
+    ap.add_argument("--form", default=None, choices=["fp32", "bf16", "bf16x3"],
+                    help="GEMM form of G and DtX (default: split-BF16x3 for C2, FP32 for --config C1/C3/C4)")
     ap.add_argument("--omp", default="qform", choices=["qform", "wave16", "wave", "mfma"],
                     help="headline Batch-OMP kernel")
     ap.add_argument("--N", type=int, default=1_000_000)
@@ -147,6 +229,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run one device copy of GIB GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4"],
+                    help="C2 (default): the headline line; C1/C3/C4: that configuration's own measurement "
+                         "(K-SVD iterations/s, error-constrained Batch-OMP signals/s, forward + adjoint signals/s)")
     ap.add_argument("--compare", action="store_true",
                     help="only run the fMRI comparison: CPU (FP32 port) vs GPU K-SVD on one problem, then exit")
     ap.add_argument("--compare-x", default=None, help="MAT-file with variable X for --compare (default: synthetic)")
@@ -155,6 +240,10 @@ def main():
     args = ap.parse_args()
     if args.compare:
         return compare(args)
+    if args.config != "C2":
+        args.form = args.form or "fp32"
+        return other_config(args)
+    args.form = args.form or "bf16x3"
     if args.copy_probe > 0:
         import torch
         x = torch.ones(int(args.copy_probe * (1 << 28)), device="cuda")

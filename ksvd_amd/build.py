@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libksvd_amd.so")
-SOURCES = ["omp_qform_t64.hip", "omp_qform_t32.hip", "omp_qform_t16.hip", "omp_qform_t8.hip",
+SOURCES = ["omp_qform_t64.hip", "omp_qform_t48.hip", "omp_qform_t32.hip", "omp_qform_t16.hip", "omp_qform_t8.hip",
            "gemm.hip", "omp.hip", "omp_qform.hip", "omp_fast.hip", "dict_update.hip", "adjoint.hip", "capi.cpp",
            "matio.cpp"]
 MAIN = os.path.join(HERE, "ksvd_main")  # CLI driver (ksvd_main.cpp) linked against the library
@@ -34,7 +34,7 @@ FLAGS = [
 # compile-time constant; LLVM's default size limit for `#pragma unroll` would
 # leave the loop rolled and push the basis to scratch memory.
 UNROLL = ["-mllvm", "-pragma-unroll-threshold=500000"]
-EXTRA = {f"omp_qform_t{tt}.hip": UNROLL for tt in (8, 16, 32, 64)}
+EXTRA = {f"omp_qform_t{tt}.hip": UNROLL for tt in (8, 16, 32, 48, 64)}
 
 
 def _compile(src: str) -> str:

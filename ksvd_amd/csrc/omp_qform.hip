@@ -8,12 +8,13 @@ namespace ksvd {
 hipError_t launch_qform_t8(const OmpArgs& a, const QformShape& sh, hipStream_t st);
 hipError_t launch_qform_t16(const OmpArgs& a, const QformShape& sh, hipStream_t st);
 hipError_t launch_qform_t32(const OmpArgs& a, const QformShape& sh, hipStream_t st);
+hipError_t launch_qform_t48(const OmpArgs& a, const QformShape& sh, hipStream_t st);
 hipError_t launch_qform_t64(const OmpArgs& a, const QformShape& sh, hipStream_t st);
 
 QformShape qform_shape(int K, int T) {
     QformShape sh{0, 0, 0};
     if (K < 1 || T < 1 || T > 64) return sh;
-    const int TT = T <= 8 ? 8 : T <= 16 ? 16 : T <= 32 ? 32 : 64;
+    const int TT = T <= 8 ? 8 : T <= 16 ? 16 : T <= 32 ? 32 : T <= 48 ? 48 : 64;
     const int kmax = (128 / TT) < 8 ? 128 / TT : 8;
     const int need = (K + 63) / 64;
     int kpl = 1;
@@ -36,6 +37,7 @@ hipError_t launch_batch_omp_qform(const OmpArgs& a, hipStream_t st) {
         case 8: return launch_qform_t8(a, sh, st);
         case 16: return launch_qform_t16(a, sh, st);
         case 32: return launch_qform_t32(a, sh, st);
+        case 48: return launch_qform_t48(a, sh, st);
         case 64: return launch_qform_t64(a, sh, st);
     }
     return hipErrorInvalidValue;

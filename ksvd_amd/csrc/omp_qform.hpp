@@ -91,7 +91,7 @@ __device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, 
 constexpr bool qform_split(int kpl, int tt, bool eps) { return kpl == 4 && tt == 32 && !eps; }
 constexpr int qform_reg_cols(int kpl, int tt, bool eps) { return qform_split(kpl, tt, eps) ? 22 : tt - 1; }
 constexpr int qform_waves_per_eu(int kpl, int tt, bool eps) {
-    return tt * kpl < 128 ? 1 : qform_split(kpl, tt, eps) ? 4 : 512 / (tt * kpl + 3 * kpl + 26);
+    return tt * kpl < 96 ? 1 : qform_split(kpl, tt, eps) ? 4 : 512 / (tt * kpl + 3 * kpl + 26);
 }
 
 // KPL lane-contiguous floats from / to LDS
@@ -277,14 +277,14 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             const float* wsrc = wrow[b][vo];
             float wv = 0.f;  // row i of L (written by wave 0)
             if (lane < i) {
-                const float* wp = wsrc + (lane & (TT - 1));
+                const float* wp = wsrc + (lane < TT ? lane : 0);
                 // rows QR <= j < i <= QT: the winner's own LDS column (its lane
                 // and slot are pos mod 64*KPL)
                 if constexpr (QL > 0)
                     if (lane >= QR) wp = &Qs[vo][lane - QR][pos & (64 * KPL - 1)];
                 wv = *wp;
             }
-            const float d = 1.f - lanes_sum_bcast<(TT + 15) / 16>(wv * wv);
+            const float d = 1.f - lanes_sum_bcast<(TT <= 16 ? 1 : TT <= 32 ? 2 : 4)>(wv * wv);
             double dd = (double)d;
             const bool redo = i > 0 && d < a.pivot_fp64;  // both FP32: exact
             if (redo) {
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
     if (v == 0) {
         wave_lds_fence();
         // gamma = L^-T y, lane-parallel backward sweep (backsubst_Lt, S/OMP.cu:126-138)
-        float rhs = (lane < i) ? ys[lane & (TT - 1)] : 0.f;
+        float rhs = (lane < i) ? ys[lane < TT ? lane : 0] : 0.f;
         for (int r = i - 1; r >= 0; --r) {
             const float gr = readlane(rhs, r) / Ls[Lp(r, r)];
             if (lane == r) sp.gam = gr;

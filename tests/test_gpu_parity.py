@@ -123,17 +123,12 @@ def test_batch_omp_bf16x3_parity_c2(o64, variant):
     say(f"bf16x3 identical-support fraction {frac:.4f}")
 
 
-@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
-def test_c2_supports_true_dictionary(o64, form):
-    """C2 sparse coding with the generating dictionary (25 dB noise: no exact
-    fits), 16384 signals, both GEMM forms of the headline: supports identical
-    to the FP64 oracle on >= 99.9 % of the signals, the rest documented ties;
-    coefficients within 1e-5."""
-    say("batch_omp C2 true dictionary", form)
-    wl = synth.make("C2", N=16384)
+def _true_dict_case(o64, N, T, form="fp32"):
+    """C2 signals coded with the generating dictionary (no exact fits)."""
+    wl = synth.make("C2", N=N)
     X = wl.X.numpy()
     D = wl.D_true.numpy().astype(np.float32)
-    N, K, n, T = X.shape[0], D.shape[0], D.shape[1], 32
+    K, n = D.shape
     eng = Engine(n, K, N, T, gemm_form=form, omp_variant="qform")
     Xg, Dg = cu(X), cu(D)
     G = eng.gram(Dg)
@@ -143,7 +138,27 @@ def test_c2_supports_true_dictionary(o64, form):
     X64, D64 = X.astype(np.float64), D.astype(np.float64)
     A064 = o64.correlate(D64, X64)
     ref = o64.batch_omp(A064, o64.gram(D64), T, gaps=True)
-    frac, rel = _check_omp((host(idx), host(val), host(cnt)), ref, A064, coef_tol=1e-5)
+    return (host(idx), host(val), host(cnt)), ref, A064
+
+
+def test_batch_omp_parity_t40_qform(o64):
+    """T = 40 (the 48-slot register basis, 8 waves of 2 atoms per lane) on C2
+    signals coded with the generating dictionary: 32 true atoms, then 8 steps
+    in the noise."""
+    say("batch_omp C2 T=40 qform, true dictionary")
+    got, ref, A064 = _true_dict_case(o64, 2048, 40)
+    _check_omp(got, ref, A064)
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
+def test_c2_supports_true_dictionary(o64, form):
+    """C2 sparse coding with the generating dictionary (25 dB noise: no exact
+    fits), 16384 signals, both GEMM forms of the headline: supports identical
+    to the FP64 oracle on >= 99.9 % of the signals, the rest documented ties;
+    coefficients within 1e-5."""
+    say("batch_omp C2 true dictionary", form)
+    got, ref, A064 = _true_dict_case(o64, 16384, 32, form)
+    frac, rel = _check_omp(got, ref, A064, coef_tol=1e-5)
     say(f"{form}: identical-support fraction {frac:.5f}")
     assert frac >= 0.999
 
