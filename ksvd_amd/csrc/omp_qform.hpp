@@ -22,7 +22,7 @@
 // of being re-streamed from L2.  All steps are unrolled, so every index into
 // Q is a compile-time constant.
 // Per step, one barrier: each wave's arg-max (DPP max + ballot) goes to LDS
-// as one 16-byte record (alpha^2 bits, atom, alpha) with its winning lane's
+// as one 12-byte record (alpha^2 bits, atom, alpha) with its winning lane's
 // row of Q (double-buffered by step parity) -> barrier -> lanes w < W of
 // every wave read the W records and reduce them by DPP (same data, same
 // order: identical decisions, so every wave leaves the loop at the same
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
     __shared__ float ys[TT];       // y = L^-1 alpha0_I
     // per step parity: each wave's candidate (alpha^2, atom, signed alpha) and
     // the candidate's row of Q (zero beyond the step), which is w if it wins
-    __shared__ int4 cand[2][W];
+    __shared__ int3 cand[2][W];
     __shared__ __align__(16) float wrow[2][W][TT];  // rows < QR used
     __shared__ __align__(16) float wfix[W][TT];  // per wave: w recomputed in FP64 (zero beyond the step)
     // columns QR .. QT-1 of Q: column j of wave v at Qs[v][j - QR][lane*KPL + m]
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
 #pragma unroll
                 for (int m = 1; m < KPL; ++m)
                     if (m == bm) av = al[m];
-                cand[b][v] = make_int4(M, kb + bm, __float_as_int(av), 0);
+                cand[b][v] = make_int3(M, kb + bm, __float_as_int(av));
                 // rows j < min(i, QR) of Q for atom kb + bm (groups of UG; rows >= i
                 // are zero); rows QR.. are read from Qs by the other waves
 #pragma unroll
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             // atom among equal values)
             int cv = INT_MIN, ck = 0, ca = 0;
             if (lane < W) {
-                const int4 r = cand[b][lane & (W - 1)];
+                const int3 r = cand[b][lane & (W - 1)];
                 cv = r.x; ck = r.y; ca = r.z;
             }
             int mx = cv;
@@ -260,7 +260,8 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             if constexpr (W >= 4) mx = max(mx, dpp_i<0x4E>(mx));
             if constexpr (W >= 8) mx = max(mx, dpp_i<0x141>(mx));
             const int best = __builtin_amdgcn_readfirstlane(mx);
-            const int vo = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(lane < W && cv == best) | (1ull << 63)) & 63;
+            // (lanes >= W hold INT_MIN, below every record)
+            const int vo = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(cv == best) | (1ull << 63)) & 63;
             const int pos = __builtin_amdgcn_readlane(ck, vo);
             const float apos = __int_as_float(__builtin_amdgcn_readlane(ca, vo));
             // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             // 0x283424DD up to +inf)
             if ((unsigned)pos >= (unsigned)K || (unsigned)(best - 0x283424DD) > (unsigned)(0x7F800000 - 0x283424DD))
                 break;
-            if (__builtin_amdgcn_ballot_w64(lane < i && sp.ind == pos) != 0ull) break;
+            if ((__builtin_amdgcn_ballot_w64(sp.ind == pos) & ((1ull << i) - 1ull)) != 0ull) break;
             float g[KPL];
             load_slice<KPL>(G + (long)pos * K, kb, K, vec, full, g);
             // w_j = q_j(pos), one per lane j < i; d = 1 - ||w||^2 from a
