@@ -446,18 +446,18 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
     for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = dnew[t];
 }
 
-// ---- per-atom phases, n <= 1024: batched, latency-hiding forms -------------
+// ---- per-atom phases, n <= 1024: batched, latency-hiding, pipelined ---------
 // Same arithmetic as du_phase_a/b_kernel above, organised for memory-level
 // parallelism: each wave loads up to 64 of its list entries (position and
-// gamma) lane-parallel, then streams FQ residual rows at a time (one float4
-// per lane per 256 columns); 8 waves per block.  Phase A's block partials are
-// reduced by the last block in a fixed order (deterministic).  Phase B does not
-// write d_p: the next atom's phase A copies the new column in (its block 0),
-// and du_tail_kernel does it for the last atom of the sweep; nothing reads
-// d_p in between.
+// gamma) lane-parallel, then streams batches of residual rows (one float4 per
+// lane per 256 columns); 8 waves per block.  Phase A's block partials are
+// reduced by du_a_reduce_kernel in a fixed order (deterministic).  Phase B
+// does not write d_p: the next launch (du_ba_kernel's block 0) copies the new
+// column in, and du_tail_kernel does it for the last atom of the sweep;
+// nothing reads d_p in between.
 constexpr int DU_WAVES = 8;
-constexpr int FQ = 8;   // residual rows in flight per wave (phase B: rows kept for the update)
-constexpr int FQA = 8;  // residual rows in flight per wave (phase A)
+constexpr int FQ = 8;   // rows per batch in du_ba_kernel, divided by the float4 chunks per row
+constexpr int FQA = 8;  // residual rows in flight per wave in du_a_kernel (phase A of the first atom)
 constexpr int DU_RB = 16;  // workgroups of the phase A reduction
 
 __device__ __forceinline__ float4 ld4(const float* __restrict__ p, int t, int n, bool vec) {

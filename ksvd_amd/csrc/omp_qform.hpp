@@ -1,4 +1,7 @@
-// omp_qform.hip -- Batch-OMP with the Gram-domain basis held in registers.
+// omp_qform.hpp -- Batch-OMP with the Gram-domain basis held on chip: the
+// kernel template and its launchers, instantiated per basis size TT by
+// omp_qform_t{8,16,32,48,64}.hip (separate translation units compile in
+// parallel) and dispatched by omp_qform.hip.
 //
 // Same selections and results as parOMP (S/OMP.cu:254-405), computed through
 // a different but algebraically identical recurrence.  With the Cholesky
@@ -86,8 +89,10 @@ __device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, 
 // quarter of the CU's 160 KB).  The other large bases would need more LDS
 // than a CU has at four waves per SIMD (KPL = 8: 2 KB per column and wave;
 // TT = 64: 17+ columns), and error mode keeps FP64 stopping state in
-// registers as well: those stay at three waves with the whole basis in
-// registers (Q, three KPL-wide arrays and ~26 others).
+// registers as well: those keep the whole basis in registers, sized for
+// 512 / (TT*KPL + 3*KPL + 26) waves per SIMD (Q, three KPL-wide arrays and
+// ~26 others): three waves, or four for the 48-slot basis (TT*KPL = 96, a
+// few spilled registers, measured faster than three waves without spills).
 constexpr bool qform_split(int kpl, int tt, bool eps) { return kpl == 4 && tt == 32 && !eps; }
 constexpr int qform_reg_cols(int kpl, int tt, bool eps) { return qform_split(kpl, tt, eps) ? 22 : tt - 1; }
 constexpr int qform_waves_per_eu(int kpl, int tt, bool eps) {
