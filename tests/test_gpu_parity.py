@@ -123,9 +123,10 @@ def test_batch_omp_bf16x3_parity_c2(o64, variant):
     say(f"bf16x3 identical-support fraction {frac:.4f}")
 
 
-def _true_dict_case(o64, N, T, form="fp32"):
-    """C2 signals coded with the generating dictionary (no exact fits)."""
-    wl = synth.make("C2", N=N)
+def _true_dict_case(o64, N, T, form="fp32", gen=None):
+    """C2 signals coded with the generating dictionary (no exact fits); `gen`
+    overrides the generator's n, K, T."""
+    wl = synth.make("C2", N=N, **(gen or {}))
     X = wl.X.numpy()
     D = wl.D_true.numpy().astype(np.float32)
     K, n = D.shape
@@ -139,6 +140,17 @@ def _true_dict_case(o64, N, T, form="fp32"):
     A064 = o64.correlate(D64, X64)
     ref = o64.batch_omp(A064, o64.gram(D64), T, gaps=True)
     return (host(idx), host(val), host(cnt)), ref, A064
+
+
+@pytest.mark.parametrize("n,K,T", [(128, 512, 32), (256, 2048, 32), (96, 1000, 24), (128, 700, 48),
+                                   (64, 130, 12)])
+def test_batch_omp_qform_shapes(o64, n, K, T):
+    """Register-basis shapes beyond the configurations: LDS-held columns at 2
+    and 8 waves per signal (K = 512, 2048, T = 32), padding lanes and per-lane
+    G-column loads (K = 1000, 700, 130), the 48-slot basis at 8 waves."""
+    say("batch_omp qform shape", n, K, T)
+    got, ref, A064 = _true_dict_case(o64, 1024, T, gen=dict(n=n, K=K, T=T))
+    _check_omp(got, ref, A064)
 
 
 def test_batch_omp_parity_t40_qform(o64):
