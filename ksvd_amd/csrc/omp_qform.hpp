@@ -396,11 +396,13 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             if (lane == r) sp.gam = gr;
             if (lane < r) rhs = rhs - Ls[Lp(r, lane)] * gr;
         }
+        // streaming stores: the results are not re-read by this kernel and
+        // should not displace G in L2
         if (a.idx && lane < T) {
-            a.idx[s * T + lane] = (lane < i) ? sp.ind : -1;
-            a.val[s * T + lane] = (lane < i) ? sp.gam : 0.f;
+            __builtin_nontemporal_store((lane < i) ? sp.ind : -1, a.idx + s * T + lane);
+            __builtin_nontemporal_store((lane < i) ? sp.gam : 0.f, a.val + s * T + lane);
         }
-        if (a.cnt && lane == 0) a.cnt[s] = i;
+        if (a.cnt && lane == 0) __builtin_nontemporal_store(i, a.cnt + s);
         if (a.c && lane < i) a.c[s * T + lane] = sp.gam;
         if (a.stats && lane == 0 && n_redo) atomicAdd(a.stats + 1, (unsigned long long)n_redo);
     }
