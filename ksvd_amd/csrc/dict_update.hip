@@ -126,6 +126,37 @@ __global__ __launch_bounds__(1024) void du_scan_ptr_kernel(const int32_t* __rest
 constexpr int32_t kPrevFlag = (int32_t)0x80000000;
 __device__ __forceinline__ int32_t entry_pos(int32_t e) { return e & 0x7FFFFFFF; }
 
+// Places the entries of signal s (count cc, lane j < cc holds slot j: atom kk,
+// coefficient vl) at the atoms' LDS cursors.
+__device__ __forceinline__ void du_fill_place(long s, int lane, int cc, int kk, float vl, int T, int K,
+                                              int32_t* __restrict__ cur, int32_t* __restrict__ rk,
+                                              int32_t* __restrict__ at, int32_t* __restrict__ list_pos,
+                                              float* __restrict__ list_g, float* __restrict__ list_nx) {
+    if (list_nx) {
+        const int r = (lane < cc) ? rk[kk] : 0;
+        if (lane < cc) at[r] = lane + 1;
+        wave_lds_fence();
+        const int pv = (lane < cc && r > 0) ? at[r - 1] : 0;
+        const int nx = (lane < cc && r + 1 < K) ? at[r + 1] : 0;
+        const float gx = __shfl(vl, nx > 0 ? nx - 1 : 0);
+        wave_lds_fence();
+        if (lane < cc) {
+            at[r] = 0;
+            const int p = cur[kk];
+            cur[kk] = p + 1;
+            list_pos[p] = (int32_t)(s * T + lane) | (pv ? kPrevFlag : 0);
+            list_g[p] = vl;
+            list_nx[p] = nx ? gx : 0.f;
+        }
+    } else if (lane < cc) {
+        const int p = cur[kk];
+        cur[kk] = p + 1;
+        list_pos[p] = (int32_t)(s * T + lane);
+        if (list_g) list_g[p] = vl;
+    }
+    wave_lds_fence();
+}
+
 __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__ idx, const int32_t* __restrict__ cnt,
                                                      int N, int T, int K, int CH,
                                                      const int32_t* __restrict__ chunk_off,
@@ -147,40 +178,29 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
     }
     wave_lds_fence();
     const long s0 = c * CH;
-    const long s1 = min((long)N, s0 + CH);
-    // software pipeline: fetch the next signal's slot while placing this one
-    int nc = (s0 < s1) ? cnt[s0] : 0;
-    int nk = (s0 < s1 && lane < nc) ? idx[s0 * T + lane] : 0;
-    for (long s = s0; s < s1; ++s) {
-        const int cc = nc, kk = nk;
-        if (s + 1 < s1) {
-            nc = cnt[s + 1];
-            nk = (lane < nc) ? idx[(s + 1) * T + lane] : 0;
+    const int ns = (int)max(0L, min((long)N, s0 + CH) - s0);
+    // FB signals per batch: their counts, slots and values are fetched with
+    // independent loads (all T slots, masked by the count afterwards), so a
+    // batch costs one memory latency instead of one per signal
+    constexpr int FB = 16;
+    for (int b = 0; b < ns; b += FB) {
+        const int cl = (lane < FB && b + lane < ns) ? cnt[s0 + b + lane] : 0;
+        int kq[FB];
+        float vq[FB];
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+            const bool ok = b + u < ns && lane < T;
+            kq[u] = ok ? idx[(s0 + b + u) * T + lane] : 0;
+            vq[u] = (ok && list_g) ? val[(s0 + b + u) * T + lane] : 0.f;
         }
-        if (list_nx) {
-            const float vl = (lane < cc) ? val[s * T + lane] : 0.f;
-            const int r = (lane < cc) ? rk[kk] : 0;
-            if (lane < cc) at[r] = lane + 1;
-            wave_lds_fence();
-            const int pv = (lane < cc && r > 0) ? at[r - 1] : 0;
-            const int nx = (lane < cc && r + 1 < K) ? at[r + 1] : 0;
-            const float gx = __shfl(vl, nx > 0 ? nx - 1 : 0);
-            wave_lds_fence();
-            if (lane < cc) {
-                at[r] = 0;
-                const int p = cur[kk];
-                cur[kk] = p + 1;
-                list_pos[p] = (int32_t)(s * T + lane) | (pv ? kPrevFlag : 0);
-                list_g[p] = vl;
-                list_nx[p] = nx ? gx : 0.f;
-            }
-        } else if (lane < cc) {
-            const int p = cur[kk];
-            cur[kk] = p + 1;
-            list_pos[p] = (int32_t)(s * T + lane);
-            if (list_g) list_g[p] = val[s * T + lane];
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+            if (b + u >= ns) break;
+            const long s = s0 + b + u;
+            const int cc = readlane(cl, u);
+            const int kk = (lane < cc) ? kq[u] : 0;
+            du_fill_place(s, lane, cc, kk, vq[u], T, K, cur, rk, at, list_pos, list_g, list_nx);
         }
-        wave_lds_fence();
     }
 }
 
@@ -847,14 +867,16 @@ __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const 
 }
 
 // R = X - D Gamma for n <= 1024: one wave per signal, each lane owns 4
-// consecutive rows per 256 (float4), 4 dictionary columns in flight; the
-// signal's own row of X is fetched first so its latency overlaps the gathers.
+// consecutive rows per 256 (float4), RES_COLS dictionary columns in flight
+// (8 measured slower than 4 at n = 256); the signal's own row of X is fetched
+// first so its latency overlaps the gathers.
 template <int NCH>
 __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restrict__ X, const float* __restrict__ D,
                                                            const int32_t* __restrict__ idx,
                                                            const float* __restrict__ val,
                                                            const int32_t* __restrict__ cnt, int n, int N, int T,
                                                            float* __restrict__ R) {
+    constexpr int RES_COLS = 4;
     const int lane = threadIdx.x & 63;
     const bool vec = (n & 3) == 0;
     for (long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6); s < N; s += (long)gridDim.x * 4) {
@@ -867,11 +889,11 @@ __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restri
             x4[ch] = ld4(X + s * n, 256 * ch + 4 * lane, n, vec);
             xh[ch] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        for (int j = 0; j < c; j += 4) {
-            float4 d4[4][NCH];
-            float g[4];
+        for (int j = 0; j < c; j += RES_COLS) {
+            float4 d4[RES_COLS][NCH];
+            float g[RES_COLS];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < RES_COLS; ++u) {
                 const bool ok = j + u < c;
                 const long col = readlane(il, ok ? j + u : j);
                 g[u] = ok ? readlane(vl, j + u) : 0.f;
@@ -880,7 +902,7 @@ __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restri
                     d4[u][ch] = ok ? ld4(D + col * n, 256 * ch + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < RES_COLS; ++u)
 #pragma unroll
                 for (int ch = 0; ch < NCH; ++ch) xh[ch] = f4fma(g[u], d4[u][ch], xh[ch]);
         }
