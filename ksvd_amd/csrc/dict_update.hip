@@ -880,9 +880,11 @@ __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restri
     const int lane = threadIdx.x & 63;
     const bool vec = (n & 3) == 0;
     for (long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6); s < N; s += (long)gridDim.x * 4) {
+        // all T slots are fetched together with the count (only lanes below
+        // the count are read), so the loads do not wait on each other
         const int c = cnt[s];
-        const int il = (lane < c) ? idx[s * T + lane] : 0;
-        const float vl = (lane < c) ? val[s * T + lane] : 0.f;
+        const int il = (lane < T) ? idx[s * T + lane] : 0;
+        const float vl = (lane < T) ? val[s * T + lane] : 0.f;
         float4 x4[NCH], xh[NCH];
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
@@ -992,19 +994,21 @@ __global__ __launch_bounds__(1024) void error_finish_kernel(const double* __rest
 
 // ---- clearDict: S/KSVD.cu:138-226 ------------------------------------------
 // usecount_k = #{s : gamma_ks^2 > 1e-7}  (use_count_kernel, S/KSVD.cu:403-415):
-// per-block LDS histogram over a grid-stride range of signals, then one
-// global add per atom and block (integer counts: order independent).
+// per-block LDS histogram over a grid-stride range of compact entries (thread
+// = entry s*T + j, so the idx/val loads are coalesced), then one global add
+// per atom and block (integer counts: order independent).
 __global__ __launch_bounds__(256) void usecount_kernel(const int32_t* __restrict__ idx, const float* __restrict__ val,
                                                        const int32_t* __restrict__ cnt, int N, int T, int K,
                                                        int32_t* __restrict__ usecount) {
     extern __shared__ int32_t hist[];
     for (int k = threadIdx.x; k < K; k += blockDim.x) hist[k] = 0;
     __syncthreads();
-    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < N; s += (long)gridDim.x * blockDim.x) {
-        const int c = cnt[s];
-        for (int j = 0; j < c; ++j) {
-            const float v = val[s * T + j];
-            if (v * v > 1e-7f) atomicAdd(&hist[idx[s * T + j]], 1);
+    const long NT = (long)N * T;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < NT; e += (long)gridDim.x * blockDim.x) {
+        const long s = e / T;
+        if ((int)(e - s * T) < cnt[s]) {
+            const float v = val[e];
+            if (v * v > 1e-7f) atomicAdd(&hist[idx[e]], 1);
         }
     }
     __syncthreads();
@@ -1230,7 +1234,7 @@ hipError_t launch_error_from_residual(DuWork& w, float* err, hipStream_t st) {
 hipError_t launch_clear_dict(DuWork& w, const float* X, float* D, const int32_t* idx, const float* val,
                              const int32_t* cnt, float* err, hipStream_t st) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(w.K, 256)), dim3(256), 0, st, w.usecount, (long)w.K, 0);
-    const unsigned ub = (unsigned)std::min<long>(blocks_for(w.N, 256), 512);
+    const unsigned ub = (unsigned)std::min<long>(blocks_for((long)w.N * w.T, 256), 512);
     hipLaunchKernelGGL(usecount_kernel, dim3(ub), dim3(256), (size_t)w.K * 4, st, idx, val, cnt, w.N, w.T, w.K,
                        w.usecount);
     hipError_t e = launch_at_times_b(D, w.n, w.K, D, w.K, w.Gtmp, 0, st);

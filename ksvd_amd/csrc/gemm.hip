@@ -92,8 +92,12 @@ __device__ __forceinline__ void put_tile(float* __restrict__ s, const float4* v)
     }
 }
 
+// Three workgroups per CU: the split form needs 192 registers per lane at
+// the compiler's default and would run two (3.03 ms against 2.70 ms for the
+// n=256, K=1024, N=10^6 correlation, with a few spilled registers); the FP32
+// and BF16 forms fit three either way.
 template <int FORM>
-__global__ __launch_bounds__(256) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
                                                          const float* __restrict__ B, long colsB,
                                                          float* __restrict__ C, int tilesA) {
     // FP32: tiles in LDS as FP32; BF16 forms: as bf16 hi (and lo)
