@@ -35,20 +35,25 @@ constexpr int PH = 40;     // LDS row pitch in bf16 elements for the BF16 forms 
 // One 128 x 32 tile (rows of the tile are columns of the matrix): each
 // thread fetches 4 float4s into registers (fetch_tile), later stored to LDS
 // (put_tile), so the next chunk's global loads are in flight while the MFMAs
-// of the current one run.
-__device__ __forceinline__ void fetch_tile(float4* v, const float* __restrict__ M, int rows, long col0, long cols,
-                                           int t0, bool vec) {
+// of the current one run.  Mt is the tile's first column (uniform), thread
+// rows are tid/8 + 32 r at 32-bit offsets from it, and `full` (uniform: whole
+// tile in range, rows a multiple of 4 and of BK) drops every bounds test.
+__device__ __forceinline__ void fetch_tile(float4* v, const float* __restrict__ Mt, int rows, int nvalid, int t0,
+                                           bool vec, bool full) {
     const int tid = threadIdx.x;
+    const int row0 = tid >> 3;
+    const int t = t0 + (tid & 7) * 4;
+    if (full) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = *reinterpret_cast<const float4*>(Mt + ((row0 + 32 * r) * rows + t));
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int q = tid + 256 * r;
-        const int row = q >> 3;
-        const int c4 = (q & 7) * 4;
-        const long col = col0 + row;
-        const int t = t0 + c4;
+        const int row = row0 + 32 * r;
         v[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (col < cols) {
-            const float* src = M + col * (long)rows + t;
+        if (row < nvalid) {
+            const float* src = Mt + (row * rows + t);
             if (vec && t + 3 < rows) {
                 v[r] = *reinterpret_cast<const float4*>(src);
             } else {
@@ -124,6 +129,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const int w = threadIdx.x >> 6;
     const int wi = w & 1, wj = w >> 1;
     const bool vec = (rows & 3) == 0;
+    const int nB = (int)min((long)BM, colsB - j0), nA = min(BN, colsA - m0);
+    const bool full = vec && rows % BK == 0 && nB == BM && nA == BN;
+    const float* Bt = B + j0 * (long)rows;
+    const float* At = A + (long)m0 * rows;
 
     f32x16 acc[2][2];
 #pragma unroll
@@ -136,8 +145,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const int li = lane & 31;
     const int lk = lane >> 5;
     float4 pb[4], pa[4];
-    fetch_tile(pb, B, rows, j0, colsB, 0, vec);
-    fetch_tile(pa, A, rows, m0, colsA, 0, vec);
+    fetch_tile(pb, Bt, rows, nB, 0, vec, full);
+    fetch_tile(pa, At, rows, nA, 0, vec, full);
     for (int t0 = 0; t0 < rows; t0 += BK) {
         if constexpr (FORM == 0) {
             put_tile(sB, pb);
@@ -148,8 +157,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         }
         __syncthreads();
         if (t0 + BK < rows) {
-            fetch_tile(pb, B, rows, j0, colsB, t0 + BK, vec);
-            fetch_tile(pa, A, rows, m0, colsA, t0 + BK, vec);
+            fetch_tile(pb, Bt, rows, nB, t0 + BK, vec, full);
+            fetch_tile(pa, At, rows, nA, t0 + BK, vec, full);
         }
         if constexpr (FORM == 0) {
 #pragma unroll
@@ -197,22 +206,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     }
 
     // Accumulator register r of lane l holds MFMA row 8*(r/4) + 4*(l/32) + r%4
-    // (a signal) and column l%32 (an atom).
+    // (a signal) and column l%32 (an atom); offsets from the tile's corner.
+    float* Ct = C + j0 * (long)colsA + m0;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            const int m = m0 + wj * 64 + b * 32 + li;
-            if (m >= colsA) continue;
+            const int ml = wj * 64 + b * 32 + li;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const long j = j0 + wi * 64 + a * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
-                if (j < colsB) C[j * (long)colsA + m] = acc[a][b][r];
+                const int jl = wi * 64 + a * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
+                if (full || (ml < nA && jl < nB)) Ct[jl * colsA + ml] = acc[a][b][r];
             }
         }
 }
 
-// normcols: one wave per atom.
 __global__ __launch_bounds__(256) void normcols_kernel(const float* __restrict__ Din, float* __restrict__ Dout,
                                                        int n, int K) {
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -232,6 +240,8 @@ __global__ __launch_bounds__(256) void normcols_kernel(const float* __restrict__
 hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B, long colsB, float* C, int form,
                              hipStream_t st) {
     if (rows <= 0 || colsA <= 0 || colsB <= 0) return hipSuccess;
+    // tile-relative offsets are 32-bit: 128 columns of A, B and C
+    if ((long)rows * BM > 0x7fffffffL || (long)colsA * BM > 0x7fffffffL) return hipErrorInvalidValue;
     const int tilesA = (colsA + BN - 1) / BN;
     const long tilesB = ((colsB + BM - 1) / BM + 7) / 8 * 8;  // whole groups of 8 (one per XCD)
     const long blocks = tilesB * tilesA;
