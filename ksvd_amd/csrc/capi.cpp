@@ -136,8 +136,7 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.ptr, c.K + 1, "ptr"));
     TRY(alloc(h, &w.chunk_hist, (size_t)c.K * w.nchunks, "chunk_hist"));
     TRY(alloc(h, &w.list_pos, (size_t)c.N * c.T, "list_pos"));
-    TRY(alloc(h, &w.list_g, (size_t)c.N * c.T, "list_g"));
-    TRY(alloc(h, &w.list_nx, (size_t)c.N * c.T, "list_nx"));
+    TRY(alloc(h, &w.list_rec, (size_t)c.N * c.T, "list_rec"));
     TRY(alloc(h, &w.R, (size_t)c.n * c.N, "R"));
     TRY(alloc(h, &w.apart, (size_t)ksvd::kDuBlocks * ((c.n + 3) & ~3), "apart"));
     TRY(alloc(h, &w.bpart, (size_t)2 * ksvd::kDuBlocks, "bpart"));

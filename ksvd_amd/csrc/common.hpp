@@ -256,9 +256,10 @@ struct DuWork {
     int32_t* count;      // K
     int32_t* ptr;        // K + 1
     int32_t* chunk_hist; // K x nchunks
-    int32_t* list_pos;   // N*T
-    float* list_g;       // N*T: gamma of each list entry at list-build time
-    float* list_nx;      // N*T: gamma of the entry's signal on the next atom in the sweep order (0 if unused)
+    int32_t* list_pos;   // N*T: entry positions s*T + slot (the sweep without pipelining, n > 1024)
+    int4* list_rec;      // N*T records of the pipelined sweep: {position | previous-atom flag (bit 31),
+                         // gamma bits, bits of the signal's gamma on the next atom in the sweep order (0 if
+                         // unused), 0}
     float* R;            // n x N restricted residuals X - D Gamma
     float* apart;        // kDuBlocks x (n rounded up to 4) partial a
     float* bpart;        // kDuBlocks x 2 partial arg-max (special case)

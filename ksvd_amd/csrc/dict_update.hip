@@ -118,11 +118,12 @@ __global__ __launch_bounds__(1024) void du_scan_ptr_kernel(const int32_t* __rest
 
 // One wave per chunk: signals in ascending order, lane j = slot j; the atoms of
 // one signal are distinct, so the LDS cursors never collide within a step.
-// With `list_nx` (the pipelined sweep) each entry (signal s, atom p) also
-// records, against the sweep order `perm`, whether s uses the atom updated
-// just before p (bit 31 of list_pos: its phase-B pass already covers s) and
-// the coefficient of s on the atom updated just after p (list_nx, 0 when s
-// does not use it).
+// With `list_rec` (the pipelined sweep) each entry (signal s, atom p) is one
+// 16-byte record (one store, one load): its position s*T + slot, whether s
+// uses the atom updated just before p in the sweep order `perm` (bit 31: its
+// phase-B pass already covers s), its coefficient, and the coefficient of s
+// on the atom updated just after p (0 when s does not use it).  Otherwise
+// only the position goes to list_pos.
 constexpr int32_t kPrevFlag = (int32_t)0x80000000;
 __device__ __forceinline__ int32_t entry_pos(int32_t e) { return e & 0x7FFFFFFF; }
 
@@ -131,8 +132,8 @@ __device__ __forceinline__ int32_t entry_pos(int32_t e) { return e & 0x7FFFFFFF;
 __device__ __forceinline__ void du_fill_place(long s, int lane, int cc, int kk, float vl, int T, int K,
                                               int32_t* __restrict__ cur, int32_t* __restrict__ rk,
                                               int32_t* __restrict__ at, int32_t* __restrict__ list_pos,
-                                              float* __restrict__ list_g, float* __restrict__ list_nx) {
-    if (list_nx) {
+                                              int4* __restrict__ list_rec) {
+    if (list_rec) {
         const int r = (lane < cc) ? rk[kk] : 0;
         if (lane < cc) at[r] = lane + 1;
         wave_lds_fence();
@@ -144,15 +145,13 @@ __device__ __forceinline__ void du_fill_place(long s, int lane, int cc, int kk, 
             at[r] = 0;
             const int p = cur[kk];
             cur[kk] = p + 1;
-            list_pos[p] = (int32_t)(s * T + lane) | (pv ? kPrevFlag : 0);
-            list_g[p] = vl;
-            list_nx[p] = nx ? gx : 0.f;
+            list_rec[p] = make_int4((int32_t)(s * T + lane) | (pv ? kPrevFlag : 0), __float_as_int(vl),
+                                    __float_as_int(nx ? gx : 0.f), 0);
         }
     } else if (lane < cc) {
         const int p = cur[kk];
         cur[kk] = p + 1;
         list_pos[p] = (int32_t)(s * T + lane);
-        if (list_g) list_g[p] = vl;
     }
     wave_lds_fence();
 }
@@ -162,15 +161,15 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
                                                      const int32_t* __restrict__ chunk_off,
                                                      const int32_t* __restrict__ ptr,
                                                      int32_t* __restrict__ list_pos,
-                                                     const float* __restrict__ val, float* __restrict__ list_g,
-                                                     const uint32_t* __restrict__ perm, float* __restrict__ list_nx) {
+                                                     const float* __restrict__ val,
+                                                     const uint32_t* __restrict__ perm, int4* __restrict__ list_rec) {
     extern __shared__ int32_t cur[];
     int32_t* rk = cur + K;  // sweep rank of each atom
     int32_t* at = rk + K;   // per signal: slot + 1 of its atom at each rank, 0 elsewhere
     const int lane = threadIdx.x;
     const long c = blockIdx.x;
     for (int k = lane; k < K; k += 64) cur[k] = ptr[k] + chunk_off[c * K + k];
-    if (list_nx) {
+    if (list_rec) {
         for (int k = lane; k < K; k += 64) {
             rk[perm[k]] = k;
             at[k] = 0;
@@ -191,7 +190,7 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
         for (int u = 0; u < FB; ++u) {
             const bool ok = b + u < ns && lane < T;
             kq[u] = ok ? idx[(s0 + b + u) * T + lane] : 0;
-            vq[u] = (ok && list_g) ? val[(s0 + b + u) * T + lane] : 0.f;
+            vq[u] = (ok && list_rec) ? val[(s0 + b + u) * T + lane] : 0.f;
         }
 #pragma unroll
         for (int u = 0; u < FB; ++u) {
@@ -199,7 +198,7 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
             const long s = s0 + b + u;
             const int cc = readlane(cl, u);
             const int kk = (lane < cc) ? kq[u] : 0;
-            du_fill_place(s, lane, cc, kk, vq[u], T, K, cur, rk, at, list_pos, list_g, list_nx);
+            du_fill_place(s, lane, cc, kk, vq[u], T, K, cur, rk, at, list_pos, list_rec);
         }
     }
 }
@@ -594,8 +593,7 @@ __device__ void du_special_case(int p, const float* __restrict__ X, float* __res
 template <int NCH>
 __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
-                                                   const int32_t* __restrict__ list_pos,
-                                                   const float* __restrict__ list_g, const float* __restrict__ X,
+                                                   const int4* __restrict__ list_rec, const float* __restrict__ X,
                                                    float* __restrict__ D, const float* __restrict__ R,
                                                    int32_t* __restrict__ unused, int32_t* __restrict__ usc,
                                                    int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
@@ -629,8 +627,9 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     }
     for (int base = gw; base < Np; base += 64 * GW) {
         const int mk = base + lane * GW;
-        const int pos = (mk < Np) ? entry_pos(list_pos[beg + mk]) : 0;
-        const float gl = (mk < Np) ? list_g[beg + mk] : 0.f;
+        const int4 rec = (mk < Np) ? list_rec[beg + mk] : make_int4(0, 0, 0, 0);
+        const int pos = entry_pos(rec.x);
+        const float gl = __int_as_float(rec.y);
         const int sl = pos / T;
         const int nk = min(64, (Np - base + GW - 1) / GW);
         for (int k = 0; k < nk; k += FQA) {
@@ -728,7 +727,7 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const uint32_t
 // s in I_p, e = r_s + d_p gamma_s, gamma_new = d_new^T e, r_s <- e - d_new
 // gamma_new, with d_new = a / ||a|| from du_a_reduce_kernel.  A signal the
 // next atom q also uses contributes gamma_q (r_s + d_q gamma_q) to q's a from
-// its updated residual, right here (list_nx holds gamma_q); every other
+// its updated residual, right here (the entry record holds gamma_q); every other
 // signal of I_q is unaffected by this update and contributes from R.  Block
 // partials of q's a go to apart (reduced by du_a_reduce_kernel(jj + 1)).
 // An unused atom p (N_p = 0, jj > 0; position 0 is handled by du_a_kernel)
@@ -736,8 +735,7 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const uint32_t
 template <int NCH>
 __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_t* __restrict__ perm,
                                                     const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
-                                                    const int32_t* __restrict__ list_pos,
-                                                    const float* __restrict__ list_g, const float* __restrict__ list_nx,
+                                                    const int4* __restrict__ list_rec,
                                                     const float* __restrict__ X, float* __restrict__ val,
                                                     float* __restrict__ D, float* __restrict__ R,
                                                     int32_t* __restrict__ unused, int32_t* __restrict__ usc,
@@ -781,15 +779,17 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
     const int Nmax = max(Np, Nq);
     for (int base = gw; base < Nmax; base += 64 * GW) {
         const int mk = base + lane * GW;
-        const int posb = (mk < Np) ? entry_pos(list_pos[bp + mk]) : 0;
-        const float gb = (mk < Np) ? list_g[bp + mk] : 0.f;
-        const float xb = (mk < Np) ? list_nx[bp + mk] : 0.f;
+        const int4 rb = (mk < Np) ? list_rec[bp + mk] : make_int4(0, 0, 0, 0);
+        const int posb = entry_pos(rb.x);
+        const float gb = __int_as_float(rb.y);
+        const float xb = __int_as_float(rb.z);
         const int slb = posb / T;
         const int nkb = (base < Np) ? min(64, (Np - base + GW - 1) / GW) : 0;
-        const int32_t ent = (mk < Nq) ? list_pos[bq + mk] : kPrevFlag;
+        const int4 ra = (mk < Nq) ? list_rec[bq + mk] : make_int4(kPrevFlag, 0, 0, 0);
+        const int32_t ent = ra.x;
         const int sla = entry_pos(ent) / T;
         const int sha = ent < 0;  // shared with p: added by the phase-B part
-        const float ga = (mk < Nq) ? list_g[bq + mk] : 0.f;
+        const float ga = __int_as_float(ra.y);
         const int nka = (base < Nq) ? min(64, (Nq - base + GW - 1) / GW) : 0;
         for (int k = 0; k < max(nkb, nka); k += FB) {
             float4 e[FB][NCH], r[FB][NCH];
@@ -923,13 +923,13 @@ void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hip
     // per position jj: phase B of jj fused with phase A of jj + 1, and the
     // reduction of jj + 1 (two launches per atom)
     hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
-                       w.list_pos, w.list_g, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
+                       w.list_rec, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
                        M, w.apart, w.bpart, abuf, anorm, w.tickets);
     hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.perm_dev, w.count, w.apart, abuf, anorm,
                        w.n, DU_BLOCKS);
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
-                           w.ptr, w.list_pos, w.list_g, w.list_nx, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
+                           w.ptr, w.list_rec, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
                            w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets);
         if (jj + 1 < w.K)
             hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.perm_dev, w.count,
@@ -1173,10 +1173,9 @@ hipError_t launch_du_build_lists(DuWork& w, const int32_t* idx, const int32_t* c
     hipLaunchKernelGGL(du_scan_chunks_kernel, dim3(w.K), dim3(256), 0, st, w.chunk_hist, w.nchunks, w.K, w.count);
     hipLaunchKernelGGL(du_scan_ptr_kernel, dim3(1), dim3(1024), 0, st, w.count, w.K, w.ptr);
     // the pipelined sweep (n <= 1024) also records each entry's neighbours in the sweep order
-    float* nx = (val && w.n <= 1024) ? w.list_nx : nullptr;
-    hipLaunchKernelGGL(du_fill_kernel, dim3(w.nchunks), dim3(64), (size_t)w.K * 4 * (nx ? 3 : 1), st, idx, cnt, w.N,
-                       w.T, w.K, w.chunk, w.chunk_hist, w.ptr, w.list_pos, val, val ? w.list_g : nullptr, w.perm_dev,
-                       nx);
+    int4* rec = (val && w.n <= 1024) ? w.list_rec : nullptr;
+    hipLaunchKernelGGL(du_fill_kernel, dim3(w.nchunks), dim3(64), (size_t)w.K * 4 * (rec ? 3 : 1), st, idx, cnt, w.N,
+                       w.T, w.K, w.chunk, w.chunk_hist, w.ptr, w.list_pos, val, w.perm_dev, rec);
     return hipGetLastError();
 }
 
