@@ -154,7 +154,7 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.Gtmp, (size_t)c.K * c.K, "Gtmp"));
     TRY(alloc(h, &w.rmse_part, w.rmse_parts, "rmse_part"));
     TRY(alloc(h, &w.rmse_dev, 1, "rmse"));
-    TRY(alloc(h, &w.repl_list, c.K + 1, "repl_list"));
+    TRY(alloc(h, &w.clear_flag, c.K, "clear_flag"));
     h->du_ready = true;
     HIPTRY(h, ksvd::launch_du_init(w, h->stream), "du_init");
     return KSVD_OK;

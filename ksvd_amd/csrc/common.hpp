@@ -275,7 +275,7 @@ struct DuWork {
     float* Gtmp;         // K x K (clearDict Gram)
     double* rmse_part;   // nparts
     double* rmse_dev;    // 1
-    int32_t* repl_list;  // K (clearDict replaced-in-sweep list) + 1 count
+    int32_t* clear_flag; // K: clearDict's per-atom test on the Gram at entry
     int nchunks, chunk;
     int rmse_parts;
 };

@@ -1016,22 +1016,50 @@ __global__ __launch_bounds__(256) void usecount_kernel(const int32_t* __restrict
         if (hist[k]) atomicAdd(&usecount[k], hist[k]);
 }
 
+// clearDict's test for every atom j on the Gram at entry, one wave per atom:
+// flag0_j = (max_{i != j} G_j[i]^2 > mu^2 or usecount_j < use_thresh) and not
+// replaced_j (S/KSVD.cu:148-160).  The sequential sweep uses it for as long
+// as no atom has been replaced (until then G is still the Gram at entry).
+__global__ __launch_bounds__(256) void clear_flags_kernel(const float* __restrict__ G,
+                                                          const int32_t* __restrict__ usecount,
+                                                          const int32_t* __restrict__ replaced, int K, double mu2,
+                                                          int use_thresh, int32_t* __restrict__ flag0) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= K) return;
+    float mx = 0.f;
+    for (int i = lane; i < K; i += 64) {
+        const float g = G[(long)j * K + i];
+        if (i != j) mx = fmaxf(mx, g * g);
+    }
+    mx = wave_max(mx);
+    if (lane == 0) flag0[j] = (((double)mx > mu2) || (usecount[j] < use_thresh)) && (replaced[j] == 0);
+}
+
 // Sequential sweep over atoms in one workgroup.  G holds D^T D for the D at
 // entry; when atom j is replaced, row and column j of G are recomputed so that
 // later steps see the current dictionary (the reference recomputes
-// G_j = D^T d_j with a GEMV per atom, S/KSVD.cu:164-173).
+// G_j = D^T d_j with a GEMV per atom, S/KSVD.cu:164-173).  Until the first
+// replacement the per-atom test is flag0 (held in LDS), so atoms that keep
+// their column cost no global read and no barrier.
 __global__ __launch_bounds__(1024) void clear_sweep_kernel(const float* __restrict__ X, float* __restrict__ D,
                                                            float* __restrict__ G,
                                                            const int32_t* __restrict__ usecount,
                                                            const int32_t* __restrict__ replaced,
+                                                           const int32_t* __restrict__ flag0,
                                                            float* __restrict__ err, int n, int K, int N,
                                                            double mu2, int use_thresh) {
+    extern __shared__ int32_t fl[];  // flag0, K entries
     __shared__ float rv[16];
     __shared__ int ri[16];
     __shared__ float wred[16];
     __shared__ int flag_sh;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) fl[k] = flag0[k];
+    __syncthreads();
+    int nrep = 0;  // atoms replaced so far in this sweep (uniform)
     for (int j = 0; j < K; ++j) {
+        if (nrep == 0 && !fl[j]) continue;
         // max_{i != j} G_j[i]^2   (device_max, S/ReductionKernels.cu:393-408)
         float mx = 0.f;
         for (int i = threadIdx.x; i < K; i += blockDim.x) {
@@ -1068,6 +1096,7 @@ __global__ __launch_bounds__(1024) void clear_sweep_kernel(const float* __restri
         }
         __syncthreads();
         const long best = ri[0];
+        ++nrep;
         float sq = 0.f;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const float x = X[best * n + t];
@@ -1238,8 +1267,10 @@ hipError_t launch_clear_dict(DuWork& w, const float* X, float* D, const int32_t*
                        w.usecount);
     hipError_t e = launch_at_times_b(D, w.n, w.K, D, w.K, w.Gtmp, 0, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(clear_sweep_kernel, dim3(1), dim3(1024), 0, st, X, D, w.Gtmp, w.usecount, w.replaced, err, w.n,
-                       w.K, w.N, w.mu_thresh * w.mu_thresh, w.use_thresh);
+    hipLaunchKernelGGL(clear_flags_kernel, dim3(blocks_for(w.K, 4)), dim3(256), 0, st, w.Gtmp, w.usecount,
+                       w.replaced, w.K, w.mu_thresh * w.mu_thresh, w.use_thresh, w.clear_flag);
+    hipLaunchKernelGGL(clear_sweep_kernel, dim3(1), dim3(1024), (size_t)w.K * 4, st, X, D, w.Gtmp, w.usecount,
+                       w.replaced, w.clear_flag, err, w.n, w.K, w.N, w.mu_thresh * w.mu_thresh, w.use_thresh);
     return hipGetLastError();
 }
 
