@@ -361,13 +361,16 @@ def test_dictionary_update_dense_layout(o64):
     eng.close()
 
 
-def test_error_and_clear_dict_parity(o64):
-    say("error + clear_dict")
+@pytest.mark.parametrize("pair", [0, 200])
+def test_error_and_clear_dict_parity(o64, pair):
+    """A coherent pair (pair, pair + 1): the first replacement comes at the
+    start of the sweep, or after 200 atoms tested on the entry Gram."""
+    say("error + clear_dict, coherent pair at", pair)
     X, D, idx, val, cnt, T = _du_inputs(o64, "C0", 2000, drop_atom=False)
     n, K, N = 64, 256, 2000
     D = D.copy()
-    D[1] = D[0] * 0.98 + 0.02 * D[2]
-    D[1] /= np.linalg.norm(D[1])
+    D[pair + 1] = D[pair] * 0.98 + 0.02 * D[pair + 2]
+    D[pair + 1] /= np.linalg.norm(D[pair + 1])
     eng = Engine(n, K, N, T)
     eng.du_init()
     Xg, Dg = cu(X), cu(D)
@@ -381,7 +384,7 @@ def test_error_and_clear_dict_parity(o64):
     Dr, er, flags = o64.clear_dict(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt,
                                    np.zeros(K, np.int32), err_before)
     np.testing.assert_array_equal(host(err) == 0, er == 0)
-    assert flags[0] == 1
+    assert flags[pair] == 1 and flags[:pair].sum() == 0
     np.testing.assert_allclose(host(Dg), Dr, atol=1e-5)
     eng.close()
 
