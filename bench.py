@@ -43,6 +43,30 @@ def omp_flops_per_signal(K, T):
     return K * T * (T + 1) + T * (T + 1) * (2 * T + 1) / 2
 
 
+BF16_MFMA_PEAK_TF = 2500.0
+
+
+def ksvd_iteration_roof_ms(n, K, N, T, form):
+    """Lower bound on one K-SVD iteration from its algorithmic work: the sum
+    over phases of max(bytes / HBM peak, MFMA FLOP / MFMA peak + FP32 FLOP /
+    FP32 peak).  Phases (s = 4 bytes):
+      Gram + correlation + Batch-OMP: bytes N (n s + 8T) (X in, compact
+        result out); MFMA 2nK(N + K) (x3 for the split form, on the FP32 MFMA
+        peak for the FP32 form); FP32 N F_omp(T);
+      dictionary update: bytes n s N (1 + T) + 16 T^2 N; FP32 4nTN + 4T^2 N + 4nK^2;
+      error + clearDict: bytes n s N + 8TN; FP32 2nTN."""
+    s = 4.0
+    bw = HBM_PEAK_GBS * 1e9
+    fv = FP32_PEAK_TF * 1e12
+    f_mfma = 2.0 * n * K * (N + K)
+    t_mfma = f_mfma * 3 / (BF16_MFMA_PEAK_TF * 1e12) if form == "bf16x3" else \
+        f_mfma / (BF16_MFMA_PEAK_TF * 1e12) if form == "bf16" else f_mfma / fv
+    t_omp = max(N * (n * s + 8 * T) / bw, t_mfma + N * omp_flops_per_signal(K, T) / fv)
+    t_du = max((n * s * N * (1 + T) + 16.0 * T * T * N) / bw, (4.0 * n * T * N + 4.0 * T * T * N + 4.0 * n * K * K) / fv)
+    t_err = max((n * s * N + 8.0 * T * N) / bw, 2.0 * n * T * N / fv)
+    return (t_omp + t_du + t_err) * 1000.0
+
+
 def profiled_traffic(kernel_src):
     """HBM bytes per launch of the Batch-OMP kernel from the committed counter
     summary (profiles/omp_qform_n256_k1024_t32.md), used only when that summary
@@ -384,6 +408,8 @@ def main():
                 "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (null if the kernel source changed since)",
     }
 
+    t_roof = ksvd_iteration_roof_ms(n, K, N, T, args.form)
+
     cpu = None
     if not args.no_cpu:
         S = min(args.cpu_sample, N)
@@ -415,6 +441,9 @@ def main():
                    "parallelism": "single GPU"},
         "ksvd_iters_per_s": round(1000.0 / ksvd_ms, 3) if ksvd_ms else None,
         "ksvd_ms_per_iter": round(ksvd_ms, 2) if ksvd_ms else None,
+        "ksvd_roofline": ({"t_roof_ms": round(t_roof, 2), "frac": round(t_roof / ksvd_ms, 4),
+                           "bound": "sum over phases of max(algorithmic bytes / 8 TB/s, MFMA FLOP / MFMA peak + "
+                                    "FP32 FLOP / 157.3 TFLOP/s)"} if ksvd_ms else None),
         "phase_ms": {"gram": round(float(np.mean(gram_ms)), 3), "batch_omp": round(omp_avg, 3)},
         "omp_candidates": cands,
         "omp_fallbacks": fallbacks,
