@@ -155,6 +155,7 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.rmse_part, w.rmse_parts, "rmse_part"));
     TRY(alloc(h, &w.rmse_dev, 1, "rmse"));
     TRY(alloc(h, &w.clear_flag, c.K, "clear_flag"));
+    TRY(alloc(h, &w.sweep, (size_t)2 * c.K, "sweep"));
     h->du_ready = true;
     HIPTRY(h, ksvd::launch_du_init(w, h->stream), "du_init");
     return KSVD_OK;

@@ -276,6 +276,7 @@ struct DuWork {
     double* rmse_part;   // nparts
     double* rmse_dev;    // 1
     int32_t* clear_flag; // K: clearDict's per-atom test on the Gram at entry
+    int4* sweep;         // 2K: per sweep position {p, N_p, ptr_p, q}, {N_q, ptr_q, 0, 0} (pipelined sweep)
     int nchunks, chunk;
     int rmse_parts;
 };

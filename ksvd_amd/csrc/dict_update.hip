@@ -671,15 +671,14 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
 // tree in LDS.  Writes a into abuf[jj & 1] and workgroup r's sum of a^2 into
 // anorm[(jj & 1)*DU_RB + r]; consumers form ||a|| and d_new = a/||a||
 // (S/Multiplication.cu:221).  A separate launch instead of a last-block
-// ticket keeps device-scope fences out of phase A.
-__global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const uint32_t* __restrict__ perm,
-                                                          const int32_t* __restrict__ count,
-                                                          const float* __restrict__ apart, float* __restrict__ abuf,
-                                                          float* __restrict__ anorm, int n, int G) {
+// ticket keeps device-scope fences out of phase A.  It runs for an unused
+// atom too (nothing reads a or ||a|| of an unused atom), so it starts with
+// no dependent load.
+__global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* __restrict__ apart,
+                                                          float* __restrict__ abuf, float* __restrict__ anorm, int n,
+                                                          int G) {
     __shared__ float4 red[512];
     __shared__ float wred[16];
-    const int p = (int)perm[jj];
-    if (count[p] == 0) return;
     const int np4 = (n + 3) & ~3;
     const int ng4 = np4 / 4;
     const int r = blockIdx.x;
@@ -742,16 +741,16 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
                                                     int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
                                                     int N, int T, int M, float* __restrict__ apart,
                                                     float* __restrict__ bpart, const float* __restrict__ abuf,
-                                                    const float* __restrict__ anorm, int32_t* __restrict__ ticket) {
+                                                    const float* __restrict__ anorm, int32_t* __restrict__ ticket,
+                                                    const int4* __restrict__ sweep) {
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a of the next atom
     __shared__ float rv[DU_WAVES];
     __shared__ int ri[DU_WAVES];
     __shared__ int flag_sh;
     __shared__ float wred[16];
-    const int p = (int)perm[jj];
-    const int Np = count[p];
-    const int q = (jj + 1 < K) ? (int)perm[jj + 1] : -1;
-    const int Nq = (q >= 0) ? count[q] : 0;
+    // this position's atom p and the next one q, their list lengths and offsets
+    const int4 d0 = sweep[2 * jj], d1 = sweep[2 * jj + 1];
+    const int p = d0.x, Np = d0.y, q = d0.w, Nq = d1.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int G = gridDim.x;
     const bool vec = (n & 3) == 0;
@@ -775,7 +774,7 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
             (Np > 0) ? ld4(abuf + (long)(jj & 1) * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
         dn[c] = make_float4(a4.x / nrm, a4.y / nrm, a4.z / nrm, a4.w / nrm);  // d_new = a / ||a||
     }
-    const int bp = (Np > 0) ? ptr[p] : 0, bq = (Nq > 0) ? ptr[q] : 0;
+    const int bp = d0.z, bq = d1.y;
     const int Nmax = max(Np, Nq);
     for (int base = gw; base < Nmax; base += 64 * GW) {
         const int mk = base + lane * GW;
@@ -915,6 +914,19 @@ __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restri
     }
 }
 
+// Per sweep position jj: {p, N_p, ptr_p, q, N_q, ptr_q, 0, 0} with p = perm[jj]
+// and q = perm[jj + 1] (-1, 0, 0 past the end; ptr 0 for an empty list).
+__global__ void du_sweep_desc_kernel(const uint32_t* __restrict__ perm, const int32_t* __restrict__ count,
+                                     const int32_t* __restrict__ ptr, int K, int4* __restrict__ sweep) {
+    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= K) return;
+    const int p = (int)perm[jj];
+    const int q = (jj + 1 < K) ? (int)perm[jj + 1] : -1;
+    const int Np = count[p], Nq = (q >= 0) ? count[q] : 0;
+    sweep[2 * jj] = make_int4(p, Np, Np > 0 ? ptr[p] : 0, q);
+    sweep[2 * jj + 1] = make_int4(Nq, Nq > 0 ? ptr[q] : 0, 0, 0);
+}
+
 template <int NCH>
 void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hipStream_t st) {
     float* abuf = w.dnew;                 // 2 x n: a by sweep-position parity
@@ -922,18 +934,19 @@ void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hip
     // sweep position 0: phase A (or the special case) and its reduction; then
     // per position jj: phase B of jj fused with phase A of jj + 1, and the
     // reduction of jj + 1 (two launches per atom)
+    hipLaunchKernelGGL(du_sweep_desc_kernel, dim3((w.K + 255) / 256), dim3(256), 0, st, w.perm_dev, w.count, w.ptr,
+                       w.K, w.sweep);
     hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
                        w.list_rec, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
                        M, w.apart, w.bpart, abuf, anorm, w.tickets);
-    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.perm_dev, w.count, w.apart, abuf, anorm,
-                       w.n, DU_BLOCKS);
+    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.apart, abuf, anorm, w.n, DU_BLOCKS);
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
                            w.ptr, w.list_rec, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
-                           w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets);
+                           w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets, w.sweep);
         if (jj + 1 < w.K)
-            hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.perm_dev, w.count,
-                               w.apart, abuf, anorm, w.n, DU_BLOCKS);
+            hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.apart, abuf, anorm, w.n,
+                               DU_BLOCKS);
     }
     hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n);
 }
