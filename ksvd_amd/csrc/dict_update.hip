@@ -755,7 +755,16 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
     const int G = gridDim.x;
     const bool vec = (n & 3) == 0;
     const int gw = blockIdx.x * DU_WAVES + w, GW = G * DU_WAVES;
-    if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, abuf, anorm, n);
+    // the new column of the atom at position jj - 1, one element per block
+    // (thread 0 of blocks 0 .. n-1), so no block carries it alone
+    if (jj > 0 && threadIdx.x == 0 && (int)blockIdx.x < n) {
+        const int4 dp = sweep[2 * (jj - 1)];
+        if (dp.y > 0) {
+            const float nrm = du_anorm(jj - 1, anorm);
+            for (int t = blockIdx.x; t < n; t += gridDim.x)
+                D[(long)dp.x * n + t] = abuf[(long)((jj - 1) & 1) * n + t] / nrm;
+        }
+    }
     float4 dq[NCH], acc[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
