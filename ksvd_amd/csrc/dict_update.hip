@@ -678,7 +678,6 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* _
                                                           float* __restrict__ abuf, float* __restrict__ anorm, int n,
                                                           int G) {
     __shared__ float4 red[512];
-    __shared__ float wred[16];
     const int np4 = (n + 3) & ~3;
     const int ng4 = np4 / 4;
     const int r = blockIdx.x;
@@ -697,16 +696,23 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* _
     }
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int half = 256; half >= ncs; half >>= 1) {  // pairwise tree over slices
+    for (int half = 256; half >= 64; half >>= 1) {  // pairwise tree over slices, across waves
         if ((int)threadIdx.x < half) {
             const float4 u = red[threadIdx.x], w4 = red[threadIdx.x + half];
             red[threadIdx.x] = make_float4(u.x + w4.x, u.y + w4.y, u.z + w4.z, u.w + w4.w);
         }
         __syncthreads();
     }
+    if (threadIdx.x >= 64) return;  // the same tree's levels below 64 (ncs <= 16) inside wave 0
+    float4 v = red[threadIdx.x];
+    for (int half = 32; half >= ncs; half >>= 1) {
+        const float4 w4 = make_float4(__shfl_down(v.x, half), __shfl_down(v.y, half), __shfl_down(v.z, half),
+                                      __shfl_down(v.w, half));
+        if ((int)threadIdx.x < half) v = make_float4(v.x + w4.x, v.y + w4.y, v.z + w4.z, v.w + w4.w);
+    }
     float sq = 0.f;
     if ((int)threadIdx.x < ncs && cg < c1) {
-        const float4 a4 = red[threadIdx.x];
+        const float4 a4 = v;
         const float av[4] = {a4.x, a4.y, a4.z, a4.w};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -717,7 +723,7 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* _
             }
         }
     }
-    sq = block_sum(sq, wred);
+    sq = wave_sum(sq);  // only wave 0 holds squares
     if (threadIdx.x == 0) anorm[(jj & 1) * DU_RB + r] = sq;
 }
 
