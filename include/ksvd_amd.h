@@ -160,6 +160,22 @@ int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float*
                                     const float* G, const float* eps2, int32_t* idx, float* val,
                                     int32_t* cnt);
 
+/* Allocation-free Batch-OMP from signals.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * results, bit for bit, as ksvd_batch_omp_from_signals on the first N signals
+ * of X (0 <= N <= the handle's N; compact output for those N signals).  Instead
+ * of handle scratch it forms DtX = D^T X chunk by chunk in one library-owned
+ * static device buffer of 32 MiB per GPU (held until the process ends):
+ * floor(8M / K) signals per chunk, KSVD_ERR_LIMIT if one signal's K
+ * correlations do not fit.  The call allocates nothing and keeps no
+ * statistics (ksvd_omp_stats then reports 0).  Every call on a GPU uses that
+ * GPU's one buffer: calls that overlap on one GPU (other streams, other
+ * handles, other threads) race and must be serialised by the caller.
+ * Variants KSVD_OMP_AUTO, KSVD_OMP_QFORM and KSVD_OMP_WAVE. */
+int ksvd_batch_omp_from_signals_noalloc(ksvd_handle* h, const float* D, const float* X, const float* G,
+                                        int N, int32_t* idx, float* val, int32_t* cnt);
+
 /* Diagnostics of the last Batch-OMP call (synchronises): candidates
  * re-evaluated in FP32 and steps that fell back to the full FP32 update
  * (compressed-G form), fallbacks of the MFMA form, or pivots recomputed in

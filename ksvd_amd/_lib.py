@@ -24,7 +24,7 @@ EXPORTS = [
     "ksvd_config_default", "ksvd_create", "ksvd_destroy", "ksvd_set_stream", "ksvd_last_error", "ksvd_version",
     "ksvd_sync", "ksvd_normcols", "ksvd_at_times_b", "ksvd_gram", "ksvd_correlate", "ksvd_batch_omp",
     "ksvd_batch_omp_compact", "ksvd_batch_omp_from_signals", "ksvd_batch_omp_from_signals_dense",
-    "ksvd_batch_omp_eps", "ksvd_batch_omp_eps_from_signals", "ksvd_omp_stats", "ksvd_du_init",
+    "ksvd_batch_omp_eps", "ksvd_batch_omp_eps_from_signals", "ksvd_batch_omp_from_signals_noalloc", "ksvd_omp_stats", "ksvd_du_init",
     "ksvd_dictionary_update", "ksvd_dictionary_update_dense", "ksvd_error", "ksvd_clear_dict",
     "ksvd_execute_iterations", "ksvd_perm_sequence", "ksvd_omp_adjoint", "ksvd_mat_read", "ksvd_mat_write",
     "ksvd_mat_last_error",
@@ -80,6 +80,7 @@ def load() -> C.CDLL:
     sig("ksvd_batch_omp_from_signals_dense", I, H, P, P, P, P, P)
     sig("ksvd_batch_omp_eps", I, H, P, P, P, P, P, P, P)
     sig("ksvd_batch_omp_eps_from_signals", I, H, P, P, P, P, P, P, P)
+    sig("ksvd_batch_omp_from_signals_noalloc", I, H, P, P, P, I, P, P, P)
     sig("ksvd_omp_stats", I, H, C.POINTER(C.c_int64), C.POINTER(C.c_int64))
     sig("ksvd_du_init", I, H)
     sig("ksvd_dictionary_update", I, H, P, P, P, P, P, P)

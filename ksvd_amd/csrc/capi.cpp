@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -43,9 +44,42 @@ struct ksvd_handle {
     float* gbound = nullptr;   // compressed-G form: per-atom bounds
     unsigned long long* omp_stats = nullptr;  // [0] candidates, [1] fallbacks
     int last_variant = 0;
+    bool last_noalloc = false;      // the last Batch-OMP call kept no statistics
     hipStream_t gstream = nullptr;  // private non-blocking stream for graph capture/replay
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
+
+// Device buffer of the allocation-free entry point
+// (ksvd_batch_omp_from_signals_noalloc): DtX = D^T X of one chunk of signals.
+// Every GPU the library runs on holds its own copy for the life of the
+// process.  In host code `scratch` is only a symbol: its device address on the
+// current GPU comes from hipGetSymbolAddress (scratch_ptr below).
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, resolved once per device
+// (at ksvd_create, or at the first allocation-free call on a device).
+hipError_t scratch_ptr(float** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
 
 // ||x_s||^2 per signal (one wave per signal).
 __global__ __launch_bounds__(256) void sumsq_cols_kernel(const float* __restrict__ X, int n, int N,
@@ -211,6 +245,7 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
 // Fills the wave-form arguments' compressed-G fields when that form is in use.
 int prepare_wave(ksvd_handle* h, const float* G, ksvd::OmpArgs& a) {
     const ksvd_config& c = h->cfg;
+    h->last_noalloc = false;
     TRY(alloc(h, &h->omp_stats, 2, "omp stats"));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, h->stream, h->omp_stats, 2);
     a.stats = h->omp_stats;
@@ -250,6 +285,7 @@ int omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float
             h->fast = p;
         }
         h->last_variant = KSVD_OMP_MFMA;
+        h->last_noalloc = false;
         return hip_check(h,
                          ksvd::launch_batch_omp_fast(X, D, G, c.n, c.K, c.N, c.T, eps2, idx, val, cnt,
                                                      (float)c.pivot_fp64, h->fast, h->fast_grid, h->counter,
@@ -313,7 +349,9 @@ int ksvd_create(ksvd_handle** out, const ksvd_config* cfg) {
     }
     rc = alloc(h, &h->counter, 4, "counter");
     *out = h;
-    return rc;
+    if (rc != KSVD_OK) return rc;
+    float* sp = nullptr;
+    return hip_check(h, scratch_ptr(&sp), "hipGetSymbolAddress(scratch)");
 }
 
 int ksvd_destroy(ksvd_handle* h) {
@@ -424,11 +462,52 @@ int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float*
     return omp_from_signals(h, D, X, G, eps2, idx, val, cnt);
 }
 
+int ksvd_batch_omp_from_signals_noalloc(ksvd_handle* h, const float* D, const float* X, const float* G, int N,
+                                        int32_t* idx, float* val, int32_t* cnt) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D || !X || !G || !idx || !val || !cnt)
+        return fail(h, KSVD_ERR_NULL, "batch_omp_from_signals_noalloc: null pointer");
+    const ksvd_config& c = h->cfg;
+    if (N < 0 || N > c.N)
+        return fail(h, KSVD_ERR_INVALID, "batch_omp_from_signals_noalloc: N=%d outside 0..%d (the handle's N)", N, c.N);
+    const size_t cap = kScratchBytes / sizeof(float);
+    if ((size_t)c.K > cap)
+        return fail(h, KSVD_ERR_LIMIT, "batch_omp_from_signals_noalloc: one signal's correlations (K=%d floats) exceed "
+                    "the %zu-byte scratch buffer", c.K, kScratchBytes);
+    if (!use_qform(h) && c.omp_variant != KSVD_OMP_WAVE && c.omp_variant != KSVD_OMP_AUTO)
+        return fail(h, KSVD_ERR_INVALID, "batch_omp_from_signals_noalloc: Batch-OMP variant %d needs handle scratch",
+                    c.omp_variant);
+    float* dtx = nullptr;
+    HIPTRY(h, scratch_ptr(&dtx), "hipGetSymbolAddress(scratch)");
+    // chunks of `chunk` signals: DtX of the chunk into the scratch buffer, then
+    // Batch-OMP on it (stream order keeps the next chunk's GEMM behind this
+    // chunk's Batch-OMP)
+    const long chunk = (long)(cap / (size_t)c.K);
+    const bool qf = use_qform(h);
+    h->last_variant = qf ? KSVD_OMP_QFORM : KSVD_OMP_WAVE;
+    h->last_noalloc = true;
+    for (long s0 = 0; s0 < N; s0 += chunk) {
+        const long nc = (N - s0 < chunk) ? N - s0 : chunk;
+        HIPTRY(h, ksvd::launch_at_times_b(D, c.n, c.K, X + s0 * c.n, nc, dtx, c.gemm_form, h->stream),
+               "correlate (noalloc)");
+        ksvd::OmpArgs a = omp_args(h, dtx, G);
+        a.N = (int)nc;
+        a.idx = idx + s0 * c.T;
+        a.val = val + s0 * c.T;
+        a.cnt = cnt + s0;
+        HIPTRY(h, qf ? ksvd::launch_batch_omp_qform(a, h->stream) : ksvd::launch_batch_omp(a, h->stream),
+               "batch_omp (noalloc)");
+    }
+    return KSVD_OK;
+}
+
 int ksvd_omp_stats(ksvd_handle* h, int64_t* candidates, int64_t* fallbacks) {
     if (!h) return KSVD_ERR_NULL;
     unsigned long long st[2] = {0ull, 0ull};
     int32_t fb = 0;
-    if (h->last_variant == KSVD_OMP_MFMA) {
+    if (h->last_noalloc) {
+        // the allocation-free entry point counts nothing
+    } else if (h->last_variant == KSVD_OMP_MFMA) {
         HIPTRY(h, hipMemcpyAsync(&fb, h->counter + 1, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream), "download");
     } else if (h->omp_stats) {
         HIPTRY(h, hipMemcpyAsync(st, h->omp_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream), "download");

@@ -167,6 +167,23 @@ class Engine:
                                                           _p(cnt)))
         return idx, val, cnt
 
+    def BatchOMP_noalloc(self, D, X, G, N=None, out=None):
+        """Allocation-free Batch-OMP on the first N signals of X
+        (ksvd_batch_omp_from_signals_noalloc): the engine allocates nothing;
+        outputs are the caller's (N, T) / (N,) tensors."""
+        N = self.N if N is None else int(N)
+        if out is None:
+            d = X.device
+            out = (torch.empty((N, self.T), dtype=torch.int32, device=d),
+                   torch.empty((N, self.T), dtype=torch.float32, device=d),
+                   torch.empty((N,), dtype=torch.int32, device=d))
+        idx, val, cnt = out
+        if X.shape[0] < N or idx.shape[0] < N or val.shape[0] < N or cnt.shape[0] < N:
+            raise ValueError("BatchOMP_noalloc: X and the outputs need at least N rows")
+        self._rc(self.lib.ksvd_batch_omp_from_signals_noalloc(self.h, _p(D), _p(X), _p(G), N, _p(idx), _p(val),
+                                                              _p(cnt)))
+        return idx, val, cnt
+
     def omp_stats(self):
         """(candidates re-evaluated in FP32, fallback steps) of the last Batch-OMP call."""
         c, f = C.c_int64(), C.c_int64()
