@@ -4,19 +4,22 @@ Workload (fits one GPU): the precision-form configuration, n=256 signal
 length, K=1024 atoms, N=1,000,000 signals, T=32 atoms per signal, Gaussian
 unit-norm atoms with 25 dB SNR, synthetic and seeded, resident in HBM before
 timing starts.  A step is one Batch-OMP pass over all N signals with the
-headline form: G = D^T D and DtX = D^T X as split-BF16x3 MFMA GEMMs (three
-BF16 products per term, FP32 accumulation: FP32-level accuracy), then the
-register-basis Batch-OMP kernel (one workgroup per signal holding
-Q = G(:,I) L^-T on chip, FP32 throughout); --form fp32 runs the GEMMs on FP32
-MFMA instead.  The plain FP32 form that streams G every step (FP32 GEMMs,
+headline form: G = D^T D and DtX = D^T X as split-BF16 MFMA GEMMs (operands
+split into three BF16 parts, x = hi + mid + lo, six BF16 products per term,
+FP32 accumulation: FP32-level accuracy), then the register-basis Batch-OMP
+kernel (one workgroup per signal holding Q = G(:,I) L^-T on chip, FP32
+throughout); --form bf16x3 uses the two-part split (three products, 16
+significant bits), --form fp32 FP32 MFMA.  The plain FP32 form that streams G every step (FP32 GEMMs,
 then the FP32-G wave kernel) is timed beside it, and the fraction of signals
 whose supports it matches is reported.
 
     python bench.py [--steps 10] [--warmup 2] [--gpus 1]
 
 Prints ONE JSON line: the metric, the roofline of the dominant kernel
-(the Batch-OMP kernel) and the CPU baseline (the FP32 port of the
-reference algorithm in oracle/, timed on a bounded sample of the same signals).
+(the Batch-OMP kernel), the CPU baseline (the FP32 port of the reference
+algorithm in oracle/, timed on the host cores on bounded samples of the same
+signals: a Batch-OMP pass and one K-SVD iteration) and, from the same leg, the
+headline result checked against the FP64 oracle on a seeded sample.
 """
 from __future__ import annotations
 
@@ -59,7 +62,7 @@ def ksvd_iteration_roof_ms(n, K, N, T, form):
     bw = HBM_PEAK_GBS * 1e9
     fv = FP32_PEAK_TF * 1e12
     f_mfma = 2.0 * n * K * (N + K)
-    t_mfma = f_mfma * 3 / (BF16_MFMA_PEAK_TF * 1e12) if form == "bf16x3" else \
+    t_mfma = f_mfma * {"bf16x3": 3, "bf16x6": 6}[form] / (BF16_MFMA_PEAK_TF * 1e12) if form in ("bf16x3", "bf16x6") else \
         f_mfma / (BF16_MFMA_PEAK_TF * 1e12) if form == "bf16" else f_mfma / fv
     t_omp = max(N * (n * s + 8 * T) / bw, t_mfma + N * omp_flops_per_signal(K, T) / fv)
     t_du = max((n * s * N * (1 + T) + 16.0 * T * T * N) / bw, (4.0 * n * T * N + 4.0 * T * T * N + 4.0 * n * K * K) / fv)
@@ -86,22 +89,113 @@ def profiled_traffic(kernel_src):
     return float(t.group(1))
 
 
-def cpu_baseline(X_sample, D, T, threads):
-    """FP32 port (oracle/liboracle_f32.so) of the reference Batch-OMP, OpenMP
-    over signals: G, DtX and Batch-OMP on the sample."""
+def host_info():
+    """Cores this process may use, the CPU model and whether the reference's
+    own CPU path (Matlab/fmriKSVD.m under Octave with KSVD-Box) can run."""
+    import platform
+    import shutil
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, env) if env > 0 else avail
+    octave = shutil.which("octave")
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cores_available": avail, "omp_num_threads_env": env or None,
+            "threads": threads,
+            "reference_cpu_path": ("octave found at %s, but KSVD-Box (ksvd.m) is not vendored with the reference "
+                                   "(README.md:12 links it for download): not run" % octave) if octave else
+                                  "unavailable offline: no MATLAB/Octave on the host and KSVD-Box is not vendored"}
+
+
+def cpu_baseline(X, D, gpu_result, T, omp_sample, ksvd_sample, parity_sample):
+    """The CPU leg: the FP32 port of the reference algorithm
+    (oracle/liboracle_f32.so: OpenMP over signals and over the independent
+    loops of the dictionary update, blocked GEMM) timed on the host cores,
+    and the FP64 oracle as the checker of the GPU's headline result.
+
+      * Batch-OMP signals/s: G, DtX and Batch-OMP on the first `omp_sample`
+        signals (1 warm-up run on 4096 signals first);
+      * K-SVD iterations/s: one full iteration (G, DtX, Batch-OMP, dictionary
+        update, error, clearDict) of the port on the first `ksvd_sample`
+        signals, scaled linearly to all N signals;
+      * parity: the FP64 oracle's Batch-OMP on a seeded sample of
+        `parity_sample` signals against the GPU's idx/val/cnt for them."""
     import numpy as np
 
     import oracle
+    from tests import helpers as H
+    info = host_info()
     o = oracle.load("f32")
-    used = o.set_threads(threads)
-    Xs = np.ascontiguousarray(X_sample, dtype=np.float32)
+    used = o.set_threads(info["threads"])
+    N = X.shape[0]
     Dn = np.ascontiguousarray(D, dtype=np.float32)
+    # Batch-OMP pass
+    S = min(omp_sample, N)
+    Xs = np.ascontiguousarray(X[:S], dtype=np.float32)
+    G = o.at_times_b_blocked(Dn, Dn)
+    o.batch_omp(o.at_times_b_blocked(Dn, Xs[:4096]), G, T)  # warm-up: threads, pages
     t0 = time.perf_counter()
-    G = o.gram(Dn)
-    A0 = o.correlate(Dn, Xs)
+    G = o.at_times_b_blocked(Dn, Dn)
+    A0 = o.at_times_b_blocked(Dn, Xs)
     o.batch_omp(A0, G, T)
     dt = time.perf_counter() - t0
-    return Xs.shape[0] / dt, used, dt
+    del A0
+    log(f"cpu Batch-OMP: {S} signals in {dt:.2f} s on {used} threads")
+    # one K-SVD iteration on a sample
+    Sk = min(ksvd_sample, N)
+    t0 = time.perf_counter()
+    r = o.ksvd(np.ascontiguousarray(X[:Sk]), Dn, T, 1, seed=0)
+    dtk = time.perf_counter() - t0
+    log(f"cpu K-SVD iteration: {Sk} signals in {dtk:.2f} s (rmse {r['rmse'][0]:.5f})")
+    # FP64 checker on a seeded sample of the GPU's result
+    o64 = oracle.load("f64")
+    o64.set_threads(info["threads"])
+    rng = np.random.default_rng(2024)
+    sample = np.sort(rng.choice(N, min(parity_sample, N), replace=False))
+    X64 = np.asarray(X[sample], dtype=np.float64)
+    D64 = Dn.astype(np.float64)
+    t0 = time.perf_counter()
+    A064 = o64.correlate(D64, X64)
+    ref = o64.batch_omp(A064, o64.gram(D64), T, gaps=True)
+    gi, gv, gc = (a[sample] for a in gpu_result)
+    st = {}
+    eq, tie, mis, bad = H.support_parity(gi, gc, ref["idx"], ref["cnt"], ref["gap"], 1e-4 * np.abs(A064).max(axis=1),
+                                         gv, ref["val"], st)
+    same = (gc == ref["cnt"]) & np.all(gi == ref["idx"], axis=1)
+    rel = np.linalg.norm(gv[same] - ref["val"][same], axis=1) / (np.linalg.norm(ref["val"][same], axis=1) + 1e-30)
+    ns = len(sample)
+    exact = st["precision_stops"]
+    parity = {"sample": f"{ns} signals drawn without replacement (numpy default_rng(2024)) from the {N}",
+              "identical_supports": eq, "identical_fraction": round(eq / ns, 6),
+              "identical_fraction_excluding_exact_fits": round(eq / max(1, ns - exact), 6),
+              "exact_fit_signals": exact, "documented_gap_ties": st["gap_ties"], "mismatches": mis,
+              "coef_rel_err_max": float(rel.max()), "coef_rel_err_median": float(np.median(rel)),
+              "rule": "ordered supports vs the FP64 oracle; a difference is a documented tie when the oracle's top-2 "
+                      "|alpha| gap at the first differing step is below 1e-4 max|alpha0|, or an exact fit (the "
+                      "oracle stopped on alpha^2 < 1e-14 at a signal that is an atom, the GPU went on with "
+                      "coefficients below 1e-3 of the norm)",
+              "check_s": round(time.perf_counter() - t0, 2)}
+    log(f"parity vs FP64 oracle: identical {eq}/{ns}, exact fits {exact}, gap ties {st['gap_ties']}, "
+        f"mismatches {mis}, coef rel err max {rel.max():.2e}")
+    cpu = {"value": round(S / dt, 1), "unit": "signals/s", "cores": used, "kind": "port",
+           "sample": f"Batch-OMP pass (G, DtX by blocked GEMM, Batch-OMP) over the first {S} of the {N} signals in "
+                     f"FP32, {dt:.2f} s; the port of the reference algorithm in oracle/ (liboracle_f32.so), OpenMP "
+                     f"on {used} threads",
+           "ksvd_iters_per_s": round(Sk / (N * dtk), 5),
+           "ksvd_sample": f"one K-SVD iteration of the port on the first {Sk} signals in {dtk:.2f} s, scaled "
+                          f"linearly to {N} signals",
+           "host": info}
+    return cpu, parity
 
 
 def compare(args):
@@ -124,7 +218,7 @@ def compare(args):
         X = synth.make("C1", N=N, n=n, K=K, T=T).X.numpy()
     D0 = np.random.default_rng(7).standard_normal((K, n)).astype(np.float32)  # D = randn(size(X,1), sources)
     iters = args.compare_iters
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = host_info()["threads"]
     o = oracle.load("f32")
     used = o.set_threads(threads)
     t0 = time.perf_counter()
@@ -242,14 +336,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default=None, choices=["fp32", "bf16", "bf16x3"],
-                    help="GEMM form of G and DtX (default: split-BF16x3 for C2, FP32 for --config C1/C3/C4)")
+    ap.add_argument("--form", default=None, choices=["fp32", "bf16", "bf16x3", "bf16x6"],
+                    help="GEMM form of G and DtX (default: split-BF16x6 for C2, FP32 for --config C1/C3/C4)")
     ap.add_argument("--omp", default="qform", choices=["qform", "wave16", "wave", "mfma"],
                     help="headline Batch-OMP kernel")
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--ksvd-iters", type=int, default=2)
-    ap.add_argument("--plain-steps", type=int, default=2, help="steps of the plain FP32 form (0: skip)")
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="timed steps of the plain FP32 form (default: --steps; 0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=262_144, help="signals of the CPU Batch-OMP pass")
+    ap.add_argument("--cpu-ksvd-sample", type=int, default=131_072, help="signals of the CPU K-SVD iteration")
+    ap.add_argument("--parity-sample", type=int, default=65_536,
+                    help="signals of the headline result checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run one device copy of GIB GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -267,7 +365,7 @@ def main():
     if args.config != "C2":
         args.form = args.form or "fp32"
         return other_config(args)
-    args.form = args.form or "bf16x3"
+    args.form = args.form or "bf16x6"
     if args.copy_probe > 0:
         import torch
         x = torch.ones(int(args.copy_probe * (1 << 28)), device="cuda")
@@ -339,13 +437,15 @@ def main():
     value = N / (ms_per_step / 1000.0)
     omp_avg = float(np.mean(omp_ms))
     c = cnt.cpu().numpy()
+    gpu_result = (idx.cpu().numpy(), val.cpu().numpy(), c)  # the headline result, checked below
     cands, fallbacks = eng.omp_stats()
     log(f"step {ms_per_step:.2f} ms (gemms {np.mean(gram_ms):.3f} ms, batch-omp {omp_avg:.2f} ms), "
         f"{value / 1e6:.2f} M signals/s, mean count {c.mean():.2f}, candidates {cands}, fallbacks {fallbacks}")
 
     # G-streaming wave form: FP32 GEMMs (G, DtX) + wave-per-signal Batch-OMP
     plain = None
-    if args.plain_steps > 0:
+    plain_steps = args.steps if args.plain_steps is None else args.plain_steps
+    if plain_steps > 0:
         eng_p = Engine(n, K, N, T, gemm_form="fp32", omp_variant="wave")
         A0p = A0 if A0 is not None else torch.empty((N, K), dtype=torch.float32, device=dev)
         Gp = torch.empty_like(G)
@@ -356,15 +456,17 @@ def main():
             eng_p.correlate(D, X, A0p)
             eng_p.batch_omp_compact(A0p, Gp, (ip, vp, cp))
 
-        pstep()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.plain_steps):
+        for _ in range(max(1, args.warmup)):
             pstep()
         torch.cuda.synchronize()
-        p_ms = (time.perf_counter() - t1) * 1000.0 / args.plain_steps
+        t1 = time.perf_counter()
+        for _ in range(plain_steps):
+            pstep()
+        torch.cuda.synchronize()
+        p_ms = (time.perf_counter() - t1) * 1000.0 / plain_steps
         same = (cp == cnt) & (ip == idx).all(dim=1)
-        plain = {"ms_per_step": round(p_ms, 3), "signals_per_s": round(N / (p_ms / 1000.0), 1),
+        plain = {"steps": plain_steps, "warmup": max(1, args.warmup),
+                 "ms_per_step": round(p_ms, 3), "signals_per_s": round(N / (p_ms / 1000.0), 1),
                  "headline_speedup": round(p_ms / ms_per_step, 3),
                  "identical_supports_vs_headline": round(float(same.float().mean().item()), 6)}
         log(f"plain FP32 form {p_ms:.2f} ms/step, headline speedup {p_ms / ms_per_step:.2f}x, "
@@ -393,7 +495,7 @@ def main():
     kname = {"qform": "omp_qform_kernel<4,32,4,false>", "wave16": "omp_wave16_kernel<16,false>",
              "wave": "omp_wave_kernel<16,false>", "mfma": "omp_fast_kernel<16>"}[args.omp]
     roofline = {
-        "bound": "mfma",
+        "bound": "fp32_valu",
         "achieved": round(achieved_tf, 3),
         "peak": FP32_PEAK_TF,
         "unit": "TFLOP/s",
@@ -403,25 +505,21 @@ def main():
         "kernel_ms": round(omp_avg, 3),
         "work_per_launch": f"F_omp = K T(T+1) + T(T+1)(2T+1)/2 = {omp_flops_per_signal(K, T):.0f} FP32 FLOP per "
                            f"signal at T={T}, summed over the {N} signals' support sizes",
-        "note": "FP32 vector FMAs (per-signal basis recurrence, no shared operand for MFMA), priced at MI355X's "
-                "157.3 TFLOP/s FP32 peak; non-packed FP32 VALU issue tops out at half of that; traffic from the "
+        "note": "compute-bound on FP32 vector FMAs (per-signal basis recurrence, no operand shared across "
+                "signals for MFMA), priced at MI355X's 157.3 TFLOP/s FP32 peak; non-packed FP32 VALU issue tops "
+                "out at half of that; traffic from the "
                 "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (null if the kernel source changed since)",
     }
 
     t_roof = ksvd_iteration_roof_ms(n, K, N, T, args.form)
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
-        S = min(args.cpu_sample, N)
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-        Xs = X[:S].cpu().numpy()
+        Xh = X.cpu().numpy()
         Dh = D.cpu().numpy()
-        log(f"cpu baseline on {S} signals, {threads} threads")
-        sps, used, dt = cpu_baseline(Xs, Dh, T, threads)
-        cpu = {"value": round(sps, 1), "unit": "signals/s", "cores": used, "kind": "port",
-               "sample": f"first {S} of the {N} signals: G, DtX and Batch-OMP in FP32 "
-                         f"(oracle/liboracle_f32.so, OpenMP), {dt:.1f} s"}
-        log(f"cpu {sps:.0f} signals/s ({dt:.1f} s)")
+        cpu, parity = cpu_baseline(Xh, Dh, gpu_result, T, args.cpu_sample, args.cpu_ksvd_sample, args.parity_sample)
+        del Xh
+        log(f"cpu {cpu['value']:.0f} signals/s, {cpu['ksvd_iters_per_s']} K-SVD iterations/s")
 
     out = {
         "metric": METRIC,
@@ -448,6 +546,7 @@ def main():
         "omp_candidates": cands,
         "omp_fallbacks": fallbacks,
         "plain_fp32_form": plain,
+        "parity_vs_fp64_oracle": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

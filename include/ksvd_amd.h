@@ -49,7 +49,10 @@ enum {
 enum {
     KSVD_FORM_FP32 = 0,   /* FP32 operands, FP32 MFMA, FP32 accumulate      */
     KSVD_FORM_BF16 = 1,   /* BF16 operands on MFMA, FP32 accumulate         */
-    KSVD_FORM_BF16X3 = 2  /* split-BF16 x3: hi*hi + hi*lo + lo*hi, FP32 acc */
+    KSVD_FORM_BF16X3 = 2, /* split-BF16 x3: hi*hi + hi*lo + lo*hi, FP32 acc  */
+                          /* (operands carried to 16 significant bits)      */
+    KSVD_FORM_BF16X6 = 3  /* three-part split x = hi + mid + lo (24 bits),  */
+                          /* six BF16 products down to 2^-16, FP32 acc      */
 };
 
 /* Batch-OMP kernel variant. */

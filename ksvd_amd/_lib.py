@@ -10,8 +10,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libksvd_amd.so")
 
-FORM_FP32, FORM_BF16, FORM_BF16X3 = 0, 1, 2
-FORMS = {"fp32": FORM_FP32, "bf16": FORM_BF16, "bf16x3": FORM_BF16X3}
+FORM_FP32, FORM_BF16, FORM_BF16X3, FORM_BF16X6 = 0, 1, 2, 3
+FORMS = {"fp32": FORM_FP32, "bf16": FORM_BF16, "bf16x3": FORM_BF16X3, "bf16x6": FORM_BF16X6}
 OMP_AUTO, OMP_WAVE, OMP_MFMA, OMP_WAVE16, OMP_QFORM = 0, 1, 2, 3, 4
 OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "mfma": OMP_MFMA, "wave16": OMP_WAVE16, "qform": OMP_QFORM}
 

@@ -142,7 +142,7 @@ int check_cfg(ksvd_handle* h, const ksvd_config* c) {
     if (c->K > 2048) return fail(h, KSVD_ERR_LIMIT, "K=%d exceeds 2048 (alpha held in VGPRs)", c->K);
     if (c->n > 4096) return fail(h, KSVD_ERR_LIMIT, "n=%d exceeds 4096", c->n);
     if ((long)c->N * c->T > 0x7fffffffL) return fail(h, KSVD_ERR_LIMIT, "N*T exceeds 2^31");
-    if (c->gemm_form < 0 || c->gemm_form > 2) return fail(h, KSVD_ERR_INVALID, "gemm_form=%d", c->gemm_form);
+    if (c->gemm_form < 0 || c->gemm_form > 3) return fail(h, KSVD_ERR_INVALID, "gemm_form=%d", c->gemm_form);
     if (c->omp_variant < 0 || c->omp_variant > 4) return fail(h, KSVD_ERR_INVALID, "omp_variant=%d", c->omp_variant);
     if (c->omp_variant == KSVD_OMP_QFORM && ksvd::qform_shape(c->K, c->T).kpl == 0)
         return fail(h, KSVD_ERR_LIMIT, "register-basis Batch-OMP needs T <= 32 when K > 1024 (K=%d T=%d)", c->K, c->T);

@@ -201,7 +201,7 @@ __device__ __forceinline__ T block_sum(T v, T* red) {
 namespace ksvd {
 
 // C (colsA x colsB) = A^T B, column-major, lda = ldb = rows.  form: 0 fp32,
-// 1 bf16, 2 bf16x3.
+// 1 bf16, 2 bf16x3, 3 bf16x6.
 hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B, long colsB, float* C,
                              int form, hipStream_t st);
 hipError_t launch_normcols(const float* Din, float* Dout, int n, int K, hipStream_t st);

@@ -16,7 +16,12 @@
 //   FP32   v_mfma_f32_32x32x2_f32   (157 TF)  FP32 products, FP32 accumulate
 //   BF16   v_mfma_f32_32x32x16_bf16 (2.5 PF)  RNE-rounded BF16 operands
 //   BF16X3 three BF16 MFMAs: hi*hi + hi*lo + lo*hi with x = hi + lo, hi =
-//          bf16(x), lo = bf16(x - hi)  (Henry, Tang & Heinecke, ARITH 2019)
+//          bf16(x), lo = bf16(x - hi)  (Henry, Tang & Heinecke, ARITH 2019):
+//          x is carried to 16 significant bits, |x - hi - lo| <= 2^-16 |x|
+//   BF16X6 the three-part split of the same paper, x = hi + mid + lo (24
+//          bits, FP32's precision), six BF16 MFMAs: every product term down
+//          to 2^-16 (hi*hi, hi*mid, mid*hi, hi*lo, mid*mid, lo*hi); the
+//          terms left out (mid*lo, lo*mid, lo*lo) are below 2^-23 relative
 // Operands are fetched as FP32; the BF16 forms convert them once, on their way
 // into LDS (bf16 hi, and lo for the split form), so one kernel serves all
 // three forms.
@@ -66,25 +71,34 @@ __device__ __forceinline__ void fetch_tile(float4* v, const float* __restrict__ 
     }
 }
 
-// BF16 forms: the tile converted once, at the store to LDS (hi = bf16(x), and
-// for the split form lo = bf16(x - hi)).
+// BF16 forms: the tile converted once, at the store to LDS (hi = bf16(x); for
+// the split forms lo = bf16(x - hi), or mid = bf16(x - hi) and
+// lo = bf16(x - hi - mid); the differences are exact in FP32).
 template <int FORM>
-__device__ __forceinline__ void put_tile_bf16(__bf16* __restrict__ sh, __bf16* __restrict__ sl, const float4* v) {
+__device__ __forceinline__ void put_tile_bf16(__bf16* __restrict__ sh, __bf16* __restrict__ sl,
+                                              __bf16* __restrict__ sm, const float4* v) {
     const int tid = threadIdx.x;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int q = tid + 256 * r;
         const float x[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
-        bf16x4 h, l;
+        bf16x4 h, l, m;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const __bf16 he = (__bf16)x[e];
             h[e] = he;
-            if constexpr (FORM == 2) l[e] = (__bf16)(x[e] - (float)he);
+            const float r1 = x[e] - (float)he;
+            if constexpr (FORM == 2) l[e] = (__bf16)r1;
+            if constexpr (FORM == 3) {
+                const __bf16 me = (__bf16)r1;
+                m[e] = me;
+                l[e] = (__bf16)(r1 - (float)me);
+            }
         }
         const int off = (q >> 3) * PH + (q & 7) * 4;
         *reinterpret_cast<bf16x4*>(sh + off) = h;
-        if constexpr (FORM == 2) *reinterpret_cast<bf16x4*>(sl + off) = l;
+        if constexpr (FORM >= 2) *reinterpret_cast<bf16x4*>(sl + off) = l;
+        if constexpr (FORM == 3) *reinterpret_cast<bf16x4*>(sm + off) = m;
     }
 }
 
@@ -108,9 +122,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     // FP32: tiles in LDS as FP32; BF16 forms: as bf16 hi (and lo)
     constexpr int NF = FORM == 0 ? BM * PITCH : 1;
     constexpr int NH = FORM == 0 ? 1 : BM * PH;
-    constexpr int NL = FORM == 2 ? BM * PH : 1;
+    constexpr int NL = FORM >= 2 ? BM * PH : 1;
+    constexpr int NM = FORM == 3 ? BM * PH : 1;
     __shared__ float sB[NF], sA[NF];
-    __shared__ __align__(16) __bf16 sBh[NH], sAh[NH], sBl[NL], sAl[NL];
+    __shared__ __align__(16) __bf16 sBh[NH], sAh[NH], sBl[NL], sAl[NL], sBm[NM], sAm[NM];
     // XCD-aware tile order: workgroups are dispatched round-robin over the 8
     // XCDs (block b on XCD b % 8), each with its own L2.  Block b works on
     // XCD x = b % 8 through its r = b / 8-th tile: signal tile
@@ -152,8 +167,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             put_tile(sB, pb);
             put_tile(sA, pa);
         } else {
-            put_tile_bf16<FORM>(sBh, sBl, pb);
-            put_tile_bf16<FORM>(sAh, sAl, pa);
+            put_tile_bf16<FORM>(sBh, sBl, sBm, pb);
+            put_tile_bf16<FORM>(sAh, sAl, sAm, pa);
         }
         __syncthreads();
         if (t0 + BK < rows) {
@@ -177,23 +192,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         } else {
 #pragma unroll
             for (int ks = 0; ks < BK; ks += 16) {
-                bf16x8 ah[2], al[2], bh[2], bl[2];
+                bf16x8 ah[2], al[2], am[2], bh[2], bl[2], bm[2];
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
                     const int off = (wi * 64 + a * 32 + li) * PH + ks + lk * 8;
                     ah[a] = *reinterpret_cast<const bf16x8*>(sBh + off);
-                    if constexpr (FORM == 2) al[a] = *reinterpret_cast<const bf16x8*>(sBl + off);
+                    if constexpr (FORM >= 2) al[a] = *reinterpret_cast<const bf16x8*>(sBl + off);
+                    if constexpr (FORM == 3) am[a] = *reinterpret_cast<const bf16x8*>(sBm + off);
                 }
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
                     const int off = (wj * 64 + b * 32 + li) * PH + ks + lk * 8;
                     bh[b] = *reinterpret_cast<const bf16x8*>(sAh + off);
-                    if constexpr (FORM == 2) bl[b] = *reinterpret_cast<const bf16x8*>(sAl + off);
+                    if constexpr (FORM >= 2) bl[b] = *reinterpret_cast<const bf16x8*>(sAl + off);
+                    if constexpr (FORM == 3) bm[b] = *reinterpret_cast<const bf16x8*>(sAm + off);
                 }
 #pragma unroll
                 for (int a = 0; a < 2; ++a)
 #pragma unroll
                     for (int b = 0; b < 2; ++b) {
+                        if constexpr (FORM == 3) {  // smallest terms first
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[a], bm[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[a], bh[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bm[b], acc[a][b], 0, 0, 0);
+                        }
                         if constexpr (FORM == 2) {
                             acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
                             acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
@@ -251,6 +275,7 @@ hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B
         case 0: hipLaunchKernelGGL(at_times_b_kernel<0>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
         case 1: hipLaunchKernelGGL(at_times_b_kernel<1>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
         case 2: hipLaunchKernelGGL(at_times_b_kernel<2>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        case 3: hipLaunchKernelGGL(at_times_b_kernel<3>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -98,6 +98,53 @@ void FN(or_at_times_b)(int rows, int colsA, int colsB, const REAL* A, const REAL
 }
 
 /* ------------------------------------------------------------------------- */
+/* The same product, blocked for the CPU baseline: OpenMP over blocks of 64  */
+/* columns of B, 2 x 4 dot products per register block so that each loaded  */
+/* element feeds several products (the compiler vectorises the shared-      */
+/* dimension loop; the FP32 port is built with reassociation allowed).  It  */
+/* rounds differently from or_at_times_b; the FP32 port uses it             */
+/* (KSVD_PORT), the FP64 checker keeps the plain loop.                      */
+/* ------------------------------------------------------------------------- */
+static void FN(dot_2x4)(int rows, const REAL* restrict a0, const REAL* restrict a1, const REAL* restrict b0,
+                        const REAL* restrict b1, const REAL* restrict b2, const REAL* restrict b3, REAL* out) {
+    REAL s00 = 0, s01 = 0, s02 = 0, s03 = 0, s10 = 0, s11 = 0, s12 = 0, s13 = 0;
+    for (int t = 0; t < rows; ++t) {
+        const REAL x0 = a0[t], x1 = a1[t];
+        s00 += x0 * b0[t]; s01 += x0 * b1[t]; s02 += x0 * b2[t]; s03 += x0 * b3[t];
+        s10 += x1 * b0[t]; s11 += x1 * b1[t]; s12 += x1 * b2[t]; s13 += x1 * b3[t];
+    }
+    out[0] = s00; out[1] = s01; out[2] = s02; out[3] = s03;
+    out[4] = s10; out[5] = s11; out[6] = s12; out[7] = s13;
+}
+
+void FN(or_at_times_b_blocked)(int rows, int colsA, int colsB, const REAL* A, const REAL* B, REAL* C) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long jb = 0; jb < colsB; jb += 64) {
+        const long je = jb + 64 < colsB ? jb + 64 : colsB;
+        for (int m0 = 0; m0 < colsA; m0 += 2) {
+            const int mc = m0 + 1 < colsA ? 2 : 1;
+            const REAL* a0 = A + (size_t)m0 * rows;
+            const REAL* a1 = A + (size_t)(m0 + mc - 1) * rows;
+            for (long j0 = jb; j0 < je; j0 += 4) {
+                const int jc = j0 + 4 <= je ? 4 : (int)(je - j0);
+                const REAL* bp[4];
+                for (int b = 0; b < 4; ++b) bp[b] = B + (size_t)(j0 + (b < jc ? b : 0)) * rows;
+                REAL out[8];
+                FN(dot_2x4)(rows, a0, a1, bp[0], bp[1], bp[2], bp[3], out);
+                for (int a = 0; a < mc; ++a)
+                    for (int b = 0; b < jc; ++b) C[(size_t)(j0 + b) * colsA + m0 + a] = out[4 * a + b];
+            }
+        }
+    }
+}
+
+#ifdef KSVD_PORT
+#define AT_TIMES_B FN(or_at_times_b_blocked)
+#else
+#define AT_TIMES_B FN(or_at_times_b)
+#endif
+
+/* ------------------------------------------------------------------------- */
 /* Batch-OMP, one signal.  S/OMP.cu:254-405 (parOMP) with helpers            */
 /*   maxabs                    S/OMP.cu:201-213  (strict '>': lowest index)  */
 /*   backsubt_indexed_append_L S/OMP.cu:169-182                              */
@@ -332,6 +379,9 @@ void FN(or_dict_update)(int n, int K, int N, int T, const REAL* X, REAL* D, cons
     FN(lists_t) l = FN(build_lists)(K, N, T, idx, cnt);
     REAL* gn2 = (REAL*)calloc((size_t)K, sizeof(REAL));
     REAL* Y = (REAL*)calloc((size_t)K * n, sizeof(REAL));
+    /* the parallel loops below each give every output to one thread, which
+       sums in the serial order: the results do not depend on the thread count */
+#pragma omp parallel for schedule(dynamic, 4)
     for (int k = 0; k < K; ++k) {
         REAL sum = 0;
         for (int m = l.ptr[k]; m < l.ptr[k + 1]; ++m) sum += SQR(val[l.pos[m]]);
@@ -392,6 +442,7 @@ void FN(or_dict_update)(int n, int K, int N, int T, const REAL* X, REAL* D, cons
         }
         v[p] = gn2[p];
         /* a = Y_p - D v + d_p ||gamma_p||^2 */
+#pragma omp parallel for schedule(static) if ((long)n * K > 65536)
         for (int t = 0; t < n; ++t) {
             REAL sum = 0;
             for (int r = 0; r < K; ++r) sum += D[(size_t)r * n + t] * v[r];
@@ -401,12 +452,14 @@ void FN(or_dict_update)(int n, int K, int N, int T, const REAL* X, REAL* D, cons
         REAL nrm = 0;
         for (int t = 0; t < n; ++t) nrm += SQR(a[t]);
         nrm = sqrt(nrm);
+#pragma omp parallel for schedule(static) if ((long)n * K > 65536)
         for (int r = 0; r < K; ++r) {
             REAL sum = 0;
             for (int t = 0; t < n; ++t) sum += D[(size_t)r * n + t] * a[t];
             u[r] = sum;
         }
         /* gamma_new for every signal on the list */
+#pragma omp parallel for schedule(static) if ((long)Np * n > 65536)
         for (int m = l.ptr[p]; m < l.ptr[p + 1]; ++m) {
             long s = l.pos[m] / T;
             REAL w = 0;
@@ -507,8 +560,8 @@ int FN(or_ksvd)(int n, int K, int N, int T, const REAL* X, const REAL* Din, REAL
     uint64_t state = seed;
     for (int k = 0; k < K; ++k) perm[k] = (uint32_t)k;
     for (int it = 0; it < iters; ++it) {
-        FN(or_at_times_b)(n, K, K, Dout, Dout, G);
-        FN(or_at_times_b)(n, K, N, Dout, X, DtX);
+        AT_TIMES_B(n, K, K, Dout, Dout, G);
+        AT_TIMES_B(n, K, N, Dout, X, DtX);
         FN(or_batch_omp)(K, N, T, DtX, G, NULL, NULL, idx, val, cnt, NULL, NULL, NULL, NULL);
         FN(or_du_init)(N, K, unused, &usc, bitmap, replaced);
         FN(or_perm_shuffle)(perm, K, &state);

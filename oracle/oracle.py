@@ -62,6 +62,7 @@ class Oracle:
         self._threads = fn("or_set_threads", _I, _I)
         self._normcols = fn("or_normcols", None, _I, _I, _P, _P)
         self._atb = fn("or_at_times_b", None, _I, _I, _I, _P, _P, _P)
+        self._atb_blk = fn("or_at_times_b_blocked", None, _I, _I, _I, _P, _P, _P)
         self._omp = fn("or_batch_omp", None, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P)
         self._perm = fn("or_perm_shuffle", None, _P, _I, _P)
         self._du_init = fn("or_du_init", None, _I, _I, _P, _P, _P, _P)
@@ -100,6 +101,17 @@ class Oracle:
         assert rows == rows2
         Cm = np.empty((colsB, colsA), dtype=self.dtype)
         self._atb(rows, colsA, colsB, _ptr(A), _ptr(B), _ptr(Cm))
+        return Cm
+
+    def at_times_b_blocked(self, A, B):
+        """The same product by the cache-blocked loop (the CPU baseline's GEMM)."""
+        A = self._f(A)
+        B = self._f(B)
+        colsA, rows = A.shape
+        colsB, rows2 = B.shape
+        assert rows == rows2
+        Cm = np.empty((colsB, colsA), dtype=self.dtype)
+        self._atb_blk(rows, colsA, colsB, _ptr(A), _ptr(B), _ptr(Cm))
         return Cm
 
     def gram(self, D):

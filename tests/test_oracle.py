@@ -43,6 +43,24 @@ def test_at_times_b_brute_force_long_double(o64):
     np.testing.assert_array_equal(o64.correlate(I, X), X)
 
 
+def test_at_times_b_blocked(o64, o32):
+    """The blocked product of the CPU baseline: in FP64 (no reassociation) the
+    same sums in the same order as the plain loop, so bit-identical; in the
+    FP32 port within FP32 rounding of the long-double product."""
+    rng = np.random.default_rng(4)
+    for rows, ca, cb in [(37, 13, 29), (64, 255, 130), (256, 66, 67)]:
+        A = rng.standard_normal((ca, rows))
+        B = rng.standard_normal((cb, rows))
+        np.testing.assert_array_equal(o64.at_times_b_blocked(A, B), o64.at_times_b(A, B))
+        ref = np.einsum("mt,jt->jm", A.astype(np.longdouble), B.astype(np.longdouble)).astype(np.float64)
+        bound = np.abs(B) @ np.abs(A).T
+        C32 = o32.at_times_b_blocked(A.astype(np.float32), B.astype(np.float32))
+        A32, B32 = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+        ref32 = np.einsum("mt,jt->jm", A32.astype(np.longdouble), B32.astype(np.longdouble)).astype(np.float64)
+        assert np.all(np.abs(C32 - ref32) <= rows * 1.2e-7 * bound)
+        assert np.abs(ref32 - ref).max() < 1e-4 * bound.max()
+
+
 # ---- Batch-OMP --------------------------------------------------------------------
 def test_omp_orthonormal_dictionary_closed_form(o64):
     """G = I: alpha after each step is alpha0 with the chosen entries zeroed,
@@ -217,6 +235,42 @@ def test_du_converges_to_rank1_svd(o64):
         U, S, Vt = np.linalg.svd(E, full_matrices=False)
         assert abs(abs(U[:, 0] @ D[p]) - 1) < 1e-8
         np.testing.assert_allclose(np.abs(Gd[I, p]), S[0] * np.abs(Vt[0]), atol=1e-6)
+
+
+def test_du_single_atom_one_sweep_closed_form(o64):
+    """K = 1: the restricted error of the only atom is X itself, so one sweep is
+    d = X gamma^T / ||X gamma^T|| and gamma'_s = d^T x_s (one power-iteration
+    step), computed here directly."""
+    rng = np.random.default_rng(11)
+    n, N = 20, 50
+    X = rng.standard_normal((N, n))
+    D = H.unit_rows(rng.standard_normal((1, n)))
+    idx = np.zeros((N, 1), np.int32)
+    cnt = np.ones(N, np.int32)
+    cnt[[3, 17]] = 0  # two signals without the atom stay out of it
+    idx[[3, 17]] = -1
+    val = rng.standard_normal((N, 1)) * (cnt[:, None] > 0)
+    a = (val[:, 0] * (cnt > 0)) @ X
+    d = a / np.linalg.norm(a)
+    D1, v1, st = o64.dict_update(X, D, idx, val, cnt, np.array([0], np.uint32))
+    np.testing.assert_allclose(D1[0], d, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(v1[:, 0], np.where(cnt > 0, X @ d, 0.0), rtol=1e-12, atol=1e-14)
+
+
+def test_du_threads_do_not_change_results(o64):
+    """The parallel loops of the update give each output to one thread."""
+    X, wl = c0(500, seed=6)
+    D = o64.normcols(wl.D_init.numpy())
+    r = _omp_state(o64, X, D, 8)
+    perm = np.random.default_rng(1).permutation(D.shape[0])
+    before = o64.set_threads(0)
+    o64.set_threads(1)
+    D1, v1, _ = o64.dict_update(X, D, r["idx"], r["val"], r["cnt"], perm)
+    o64.set_threads(max(2, before))
+    D2, v2, _ = o64.dict_update(X, D, r["idx"], r["val"], r["cnt"], perm)
+    o64.set_threads(before)
+    np.testing.assert_array_equal(D1, D2)
+    np.testing.assert_array_equal(v1, v2)
 
 
 def test_du_unused_atom_replacement(o64):
