@@ -253,13 +253,47 @@ const char* ksvd_mat_last_error(void);
 
 /* ---- differentiation ----------------------------------------------------- */
 
+/* Batch-OMP (compact output) that also stores each signal's Cholesky factor:
+ * Lfac (N x T(T+1)/2 floats), L L^T = G_II of the accepted atoms in slot
+ * order, lower triangle packed column by column with pitch T -- L(r, c) at
+ * c*T - c(c-1)/2 + (r - c), zero for rows or columns at or beyond the
+ * signal's count.  The reference builds this factor (Lchol,
+ * S/OMP.cu:316-336) and drops it; here it feeds ksvd_omp_adjoint_factored.
+ * Register-basis variant only (KSVD_ERR_INVALID otherwise). */
+int ksvd_batch_omp_compact_factors(ksvd_handle* h, const float* DtX, const float* G, int32_t* idx,
+                                   float* val, int32_t* cnt, float* Lfac);
+
 /* Adjoint of the fixed-support least squares gamma_I = (D_I^T D_I)^{-1} D_I^T x
  * (no reference counterpart).  gval (T x N) = dL/dgamma in slot order.
  * Writes gX (n x N) = D_I z and gD (n x K) = sum_s (r z^T - D_I z gamma^T)
- * scattered to the support columns, z = (D_I^T D_I)^{-1} g, r = x - D_I gamma. */
+ * on the support columns (zero elsewhere), z = (D_I^T D_I)^{-1} g,
+ * r = x - D_I gamma.  Low-memory mode: G_II and its factor are recomputed
+ * per signal.  No atomics: the same inputs give the same bits. */
 int ksvd_omp_adjoint(ksvd_handle* h, const float* D, const float* X, const int32_t* idx,
                      const float* val, const int32_t* cnt, const float* gval, float* gD,
                      float* gX);
+
+/* The same adjoint reusing the factors ksvd_batch_omp_compact_factors stored
+ * (Lfac as described there) instead of recomputing G_II. */
+int ksvd_omp_adjoint_factored(ksvd_handle* h, const float* D, const float* X, const int32_t* idx,
+                              const float* val, const int32_t* cnt, const float* Lfac,
+                              const float* gval, float* gD, float* gX);
+
+/* Adjoint of ksvd_normcols: with d_k = D_in_k / ||D_in_k||,
+ * gD_in_k = (gD_out_k - d_k (d_k^T gD_out_k)) / ||D_in_k||.  The forward is
+ * KSVD::normcols (S/KSVD.cu:267-286); the derivative has no counterpart. */
+int ksvd_normcols_adjoint(ksvd_handle* h, const float* D_in, const float* gD_out, float* gD_in);
+
+/* Compact Gamma -> dense Gamma values (K x N column-major, zeros off the
+ * support): the layout sprow_kernel leaves in SELECTED_ATOMS
+ * (S/DictionaryUpdate.cu:378-395). */
+int ksvd_compact_to_dense(ksvd_handle* h, const int32_t* idx, const float* val, const int32_t* cnt,
+                          float* gamma_dense);
+
+/* The reverse gather: out (T x N) = dense(idx[s, j], s) for j < cnt[s], 0
+ * beyond (a dense K x N gradient brought to slot order). */
+int ksvd_compact_gather(ksvd_handle* h, const float* dense, const int32_t* idx, const int32_t* cnt,
+                        float* out);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,8 @@ EXPORTS = [
     "ksvd_batch_omp_eps", "ksvd_batch_omp_eps_from_signals", "ksvd_batch_omp_from_signals_noalloc", "ksvd_omp_stats", "ksvd_du_init",
     "ksvd_dictionary_update", "ksvd_dictionary_update_dense", "ksvd_error", "ksvd_clear_dict",
     "ksvd_execute_iterations", "ksvd_perm_sequence", "ksvd_omp_adjoint", "ksvd_mat_read", "ksvd_mat_write",
+    "ksvd_batch_omp_compact_factors", "ksvd_omp_adjoint_factored", "ksvd_normcols_adjoint", "ksvd_compact_to_dense",
+    "ksvd_compact_gather",
     "ksvd_mat_last_error",
 ]
 
@@ -90,6 +92,11 @@ def load() -> C.CDLL:
     sig("ksvd_execute_iterations", I, H, P, P, P, P, P, P, I, P)
     sig("ksvd_perm_sequence", I, C.c_uint64, I, I, P)
     sig("ksvd_omp_adjoint", I, H, P, P, P, P, P, P, P, P)
+    sig("ksvd_batch_omp_compact_factors", I, H, P, P, P, P, P, P)
+    sig("ksvd_omp_adjoint_factored", I, H, P, P, P, P, P, P, P, P, P)
+    sig("ksvd_normcols_adjoint", I, H, P, P, P)
+    sig("ksvd_compact_to_dense", I, H, P, P, P, P)
+    sig("ksvd_compact_gather", I, H, P, P, P, P)
     sig("ksvd_mat_read", I, C.c_char_p, C.c_char_p, P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64))
     sig("ksvd_mat_write", I, C.c_char_p, C.c_char_p, P, C.c_int64, C.c_int64, I, I)
     sig("ksvd_mat_last_error", C.c_char_p)

@@ -40,6 +40,7 @@ struct ksvd_handle {
     void* fast = nullptr;      // MFMA-form Batch-OMP scratch
     int fast_grid = 0;
     float* xnorm2 = nullptr;   // N (wave-form error mode from signals)
+    float* adj_z = nullptr;    // T x N (adjoint: z per support slot)
     _Float16* gh = nullptr;    // compressed-G form: fp16 G
     float* gbound = nullptr;   // compressed-G form: per-atom bounds
     unsigned long long* omp_stats = nullptr;  // [0] candidates, [1] fallbacks
@@ -428,6 +429,23 @@ int ksvd_batch_omp_compact(ksvd_handle* h, const float* DtX, const float* G, int
     return hip_check(h, launch_wave_family(h, a), "batch_omp_compact");
 }
 
+int ksvd_batch_omp_compact_factors(ksvd_handle* h, const float* DtX, const float* G, int32_t* idx, float* val,
+                                   int32_t* cnt, float* Lfac) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!DtX || !G || !idx || !val || !cnt || !Lfac)
+        return fail(h, KSVD_ERR_NULL, "batch_omp_compact_factors: null pointer");
+    if (!use_qform(h))
+        return fail(h, KSVD_ERR_INVALID, "batch_omp_compact_factors: the factors come from the register-basis form "
+                    "(variant %d, K=%d, T=%d does not use it)", h->cfg.omp_variant, h->cfg.K, h->cfg.T);
+    ksvd::OmpArgs a = omp_args(h, DtX, G);
+    a.idx = idx;
+    a.val = val;
+    a.cnt = cnt;
+    a.Lfac = Lfac;
+    TRY(prepare_wave(h, G, a));
+    return hip_check(h, launch_wave_family(h, a), "batch_omp_compact_factors");
+}
+
 int ksvd_batch_omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G, int32_t* idx,
                                 float* val, int32_t* cnt) {
     if (!h) return KSVD_ERR_NULL;
@@ -710,14 +728,56 @@ int ksvd_perm_sequence(uint64_t seed, int K, int iterations, uint32_t* out) {
     return KSVD_OK;
 }
 
+static int omp_adjoint(ksvd_handle* h, const float* D, const float* X, const int32_t* idx, const float* val,
+                       const int32_t* cnt, const float* Lfac, const float* gval, float* gD, float* gX) {
+    const ksvd_config& c = h->cfg;
+    TRY(ensure_du(h));
+    TRY(alloc(h, &h->adj_z, (size_t)c.N * c.T, "adjoint z"));
+    // per-atom lists (ascending signal order), as the dictionary update builds them
+    HIPTRY(h, ksvd::launch_du_build_lists(h->du, idx, cnt, h->stream), "adjoint lists");
+    return hip_check(h,
+                     ksvd::launch_omp_adjoint(D, X, idx, val, cnt, Lfac, gval, gD, gX, h->du.ptr, h->du.list_pos,
+                                              h->du.R, h->adj_z, c.n, c.K, c.N, c.T, h->stream),
+                     "omp_adjoint");
+}
+
 int ksvd_omp_adjoint(ksvd_handle* h, const float* D, const float* X, const int32_t* idx, const float* val,
                      const int32_t* cnt, const float* gval, float* gD, float* gX) {
     if (!h) return KSVD_ERR_NULL;
     if (!D || !X || !idx || !val || !cnt || !gval || !gD || !gX)
         return fail(h, KSVD_ERR_NULL, "omp_adjoint: null pointer");
+    return omp_adjoint(h, D, X, idx, val, cnt, nullptr, gval, gD, gX);
+}
+
+int ksvd_omp_adjoint_factored(ksvd_handle* h, const float* D, const float* X, const int32_t* idx, const float* val,
+                              const int32_t* cnt, const float* Lfac, const float* gval, float* gD, float* gX) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D || !X || !idx || !val || !cnt || !Lfac || !gval || !gD || !gX)
+        return fail(h, KSVD_ERR_NULL, "omp_adjoint_factored: null pointer");
+    return omp_adjoint(h, D, X, idx, val, cnt, Lfac, gval, gD, gX);
+}
+
+int ksvd_normcols_adjoint(ksvd_handle* h, const float* D_in, const float* gD_out, float* gD_in) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D_in || !gD_out || !gD_in) return fail(h, KSVD_ERR_NULL, "normcols_adjoint: null pointer");
+    return hip_check(h, ksvd::launch_normcols_adjoint(D_in, gD_out, gD_in, h->cfg.n, h->cfg.K, h->stream),
+                     "normcols_adjoint");
+}
+
+int ksvd_compact_to_dense(ksvd_handle* h, const int32_t* idx, const float* val, const int32_t* cnt,
+                          float* gamma_dense) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!idx || !val || !cnt || !gamma_dense) return fail(h, KSVD_ERR_NULL, "compact_to_dense: null pointer");
     const ksvd_config& c = h->cfg;
-    return hip_check(h, ksvd::launch_omp_adjoint(D, X, idx, val, cnt, gval, gD, gX, c.n, c.K, c.N, c.T, h->stream),
-                     "omp_adjoint");
+    return hip_check(h, ksvd::launch_compact_to_dense(idx, val, cnt, c.K, c.N, c.T, gamma_dense, h->stream),
+                     "compact_to_dense");
+}
+
+int ksvd_compact_gather(ksvd_handle* h, const float* dense, const int32_t* idx, const int32_t* cnt, float* out) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!dense || !idx || !cnt || !out) return fail(h, KSVD_ERR_NULL, "compact_gather: null pointer");
+    const ksvd_config& c = h->cfg;
+    return hip_check(h, ksvd::launch_gather_compact(dense, idx, cnt, c.K, c.N, c.T, out, h->stream), "compact_gather");
 }
 
 }  // extern "C"

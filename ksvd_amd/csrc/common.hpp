@@ -223,6 +223,8 @@ struct OmpArgs {
     const float* gbound;    // compressed-G form: per atom (g_k, d_k)
     int Kpad;
     unsigned long long* stats;  // compressed-G form: [0] candidates re-evaluated, [1] FP32 fallbacks
+    float* Lfac;                // optional N x T(T+1)/2: each signal's Cholesky factor L (L L^T = G_II),
+                                // lower triangle packed column by column (register-basis form)
 };
 hipError_t launch_batch_omp(const OmpArgs& a, hipStream_t st);
 int g16_kpad(int K);
@@ -295,9 +297,16 @@ hipError_t launch_dense_to_compact(const float* sel, const float* c, int K, int 
 hipError_t launch_compact_to_dense(const int32_t* idx, const float* val, const int32_t* cnt, int K, int N,
                                    int T, float* gamma_dense, hipStream_t st);
 
-// Adjoint (adjoint.hip)
+// Adjoint (adjoint.hip).  Lfac NULL: G_II and its factor recomputed per
+// signal; else the packed factors Batch-OMP stored.  list_ptr / list_pos: the
+// per-atom lists of (signal, slot) positions in ascending signal order
+// (launch_du_build_lists); R, zbuf: n x N and T x N scratch.
 hipError_t launch_omp_adjoint(const float* D, const float* X, const int32_t* idx, const float* val,
-                              const int32_t* cnt, const float* gval, float* gD, float* gX, int n, int K,
-                              int N, int T, hipStream_t st);
+                              const int32_t* cnt, const float* Lfac, const float* gval, float* gD, float* gX,
+                              const int32_t* list_ptr, const int32_t* list_pos, float* R, float* zbuf, int n,
+                              int K, int N, int T, hipStream_t st);
+hipError_t launch_normcols_adjoint(const float* Din, const float* gout, float* gin, int n, int K, hipStream_t st);
+hipError_t launch_gather_compact(const float* dense, const int32_t* idx, const int32_t* cnt, int K, int N, int T,
+                                 float* out, hipStream_t st);
 
 }  // namespace ksvd

@@ -405,6 +405,15 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
         if (a.cnt && lane == 0) __builtin_nontemporal_store(i, a.cnt + s);
         if (a.c && lane < i) a.c[s * T + lane] = sp.gam;
         if (a.stats && lane == 0 && n_redo) atomicAdd(a.stats + 1, (unsigned long long)n_redo);
+        if (a.Lfac) {
+            // L of the accepted atoms, packed column by column with pitch T:
+            // L(r, c) at c*T - c(c-1)/2 + (r - c); zero beyond the count
+            float* Lo = a.Lfac + s * (long)(T * (T + 1) / 2);
+            for (int c = 0; c < T; ++c) {
+                const int r = c + lane;
+                if (r < T) Lo[c * T - ((c * (c - 1)) >> 1) + lane] = (r < i && c < i) ? Ls[Lp(r, c)] : 0.f;
+            }
+        }
     }
     if (a.sel) {
         // SELECTED_ATOMS column: slot+1 of each support atom, and the atom a

@@ -239,12 +239,49 @@ class Engine:
         return D_out, idx, val, cnt, list(rmse)[:iternum]
 
     # -- differentiation ------------------------------------------------------------------
-    def omp_adjoint(self, D, X, idx, val, cnt, gval, gD=None, gX=None):
+    def omp_adjoint(self, D, X, idx, val, cnt, gval, gD=None, gX=None, Lfac=None):
+        """Fixed-support adjoint: (gD, gX).  With Lfac (from
+        batch_omp_compact_factors) the stored Cholesky factors are reused,
+        otherwise G_II is recomputed per signal."""
         gD = torch.empty_like(D) if gD is None else gD
         gX = torch.empty_like(X) if gX is None else gX
-        self._rc(self.lib.ksvd_omp_adjoint(self.h, _p(D), _p(X), _p(idx), _p(val), _p(cnt), _p(gval), _p(gD),
-                                           _p(gX)))
+        if Lfac is None:
+            self._rc(self.lib.ksvd_omp_adjoint(self.h, _p(D), _p(X), _p(idx), _p(val), _p(cnt), _p(gval), _p(gD),
+                                               _p(gX)))
+        else:
+            _check_shape(Lfac, (self.N, self.T * (self.T + 1) // 2), torch.float32, "Lfac")
+            self._rc(self.lib.ksvd_omp_adjoint_factored(self.h, _p(D), _p(X), _p(idx), _p(val), _p(cnt), _p(Lfac),
+                                                        _p(gval), _p(gD), _p(gX)))
         return gD, gX
+
+    def batch_omp_compact_factors(self, DtX, G, out=None, Lfac=None):
+        """Batch-OMP (compact output) that also returns each signal's packed
+        Cholesky factor (N, T(T+1)/2)."""
+        _check_shape(DtX, (self.N, self.K), torch.float32, "DtX")
+        idx, val, cnt = self.empty_compact() if out is None else out
+        if Lfac is None:
+            Lfac = torch.empty((self.N, self.T * (self.T + 1) // 2), dtype=torch.float32, device=DtX.device)
+        self._rc(self.lib.ksvd_batch_omp_compact_factors(self.h, _p(DtX), _p(G), _p(idx), _p(val), _p(cnt),
+                                                         _p(Lfac)))
+        return idx, val, cnt, Lfac
+
+    def normcols_adjoint(self, D_in, gD_out, gD_in=None):
+        gD_in = torch.empty_like(D_in) if gD_in is None else gD_in
+        self._rc(self.lib.ksvd_normcols_adjoint(self.h, _p(D_in), _p(gD_out), _p(gD_in)))
+        return gD_in
+
+    def compact_to_dense(self, idx, val, cnt, out=None):
+        """Dense Gamma (N, K) (the reference's K x N column-major) from the compact form."""
+        out = torch.empty((self.N, self.K), dtype=torch.float32, device=idx.device) if out is None else out
+        self._rc(self.lib.ksvd_compact_to_dense(self.h, _p(idx), _p(val), _p(cnt), _p(out)))
+        return out
+
+    def compact_gather(self, dense, idx, cnt, out=None):
+        """out (N, T) = dense[s, idx[s, j]] for j < cnt[s], else 0."""
+        _check_shape(dense, (self.N, self.K), torch.float32, "dense")
+        out = torch.empty((self.N, self.T), dtype=torch.float32, device=dense.device) if out is None else out
+        self._rc(self.lib.ksvd_compact_gather(self.h, _p(dense), _p(idx), _p(cnt), _p(out)))
+        return out
 
 
 def version() -> str:

@@ -486,29 +486,97 @@ def test_reference_fixture_structure_gpu():
 
 
 # ---- adjoint ---------------------------------------------------------------------------------
-def test_omp_adjoint_parity_and_finite_differences(o64):
-    say("omp_adjoint C4 subset")
-    wl = synth.make("C4", N=512)
+def _c4_case(o64, N=512):
+    wl = synth.make("C4", N=N)
     X = wl.X.numpy()
     D = wl.D_init.numpy()
+    return wl, X, D
+
+
+def _fd_errors(loss, D64, X64, gD, gX, idx_h, cnt_h, rng, count=64, h=1e-6):
+    """FP64 central differences of `loss` at `count` sampled entries of D (on
+    support atoms) and of X.  Returns, for D and for X, (the norm-wise
+    relative error ||fd - g|| / ||fd|| over the sampled entries, the largest
+    error of one entry relative to max(|fd|, 1e-3 of the gradient's largest
+    entry))."""
+    N, n = X64.shape
+    sD, sX = np.abs(gD).max(), np.abs(gX).max()
+    fD, vD, fX, vX = [], [], [], []
+    for _ in range(count):
+        s = int(rng.integers(N))
+        while cnt_h[s] == 0:
+            s = int(rng.integers(N))
+        k, t = int(idx_h[s, int(rng.integers(cnt_h[s]))]), int(rng.integers(n))
+        Dp, Dm = D64.copy(), D64.copy()
+        Dp[k, t] += h
+        Dm[k, t] -= h
+        fD.append((loss(Dp, X64) - loss(Dm, X64)) / (2 * h))
+        vD.append(gD[k, t])
+        s, t = int(rng.integers(N)), int(rng.integers(n))
+        Xp, Xm = X64.copy(), X64.copy()
+        Xp[s, t] += h
+        Xm[s, t] -= h
+        fX.append((loss(D64, Xp) - loss(D64, Xm)) / (2 * h))
+        vX.append(gX[s, t])
+    out = []
+    for f, v, sc in ((fD, vD, sD), (fX, vX, sX)):
+        f, v = np.asarray(f), np.asarray(v, dtype=np.float64)
+        out.append((np.linalg.norm(f - v) / np.linalg.norm(f),
+                    float(np.max(np.abs(f - v) / np.maximum(np.abs(f), 1e-3 * sc)))))
+    return out
+
+
+# adjoint vs FP64 central differences: norm-wise relative error over the
+# sampled entries, and per entry (FP32 gradients of entries far below the
+# largest one carry cancellation error relative to themselves)
+FD_TOL, FD_ENTRY_TOL = 1e-4, 1e-3
+
+
+@pytest.mark.parametrize("factored", [True, False])
+def test_omp_adjoint_parity_and_finite_differences(o64, factored):
+    """Adjoint of the fixed-support least squares (C4 subset), with the
+    factors Batch-OMP stored or recomputed: against the oracle's closed form
+    (1e-4 of the gradient's scale), against FP64 central differences on 64
+    entries of D and 64 of X (norm-wise 1e-4 relative), and bit-identical on
+    a second run (no atomics)."""
+    say("omp_adjoint C4 subset, stored factors" if factored else "omp_adjoint C4 subset, recomputed factors")
+    wl, X, D = _c4_case(o64)
     D = (D / np.linalg.norm(D.astype(np.float64), axis=1, keepdims=True)).astype(np.float32)
     n, K, N, T = 64, 256, 512, 10
     eng = Engine(n, K, N, T)
     Xg, Dg = cu(X), cu(D)
-    idx, val, cnt = eng.batch_omp_compact(eng.correlate(Dg, Xg), eng.gram(Dg))
-    W = wl.W.numpy()
-    y = wl.y.numpy()
+    A0 = eng.correlate(Dg, Xg)
+    G = eng.gram(Dg)
+    if factored:
+        idx, val, cnt, Lfac = eng.batch_omp_compact_factors(A0, G)
+    else:
+        idx, val, cnt = eng.batch_omp_compact(A0, G)
+        Lfac = None
+    W, y = wl.W.numpy(), wl.y.numpy()
     idx_h, cnt_h, val_h = host(idx), host(cnt), host(val)
     # loss ||W gamma - y||^2 with gamma the dense K-vector: g = 2 W^T (W gamma - y) on the support
     Gd = H.dense_gamma(idx_h, val_h, cnt_h, K)
-    R = Gd @ W.T - y
-    gdense = 2 * R @ W
+    gdense = 2 * (Gd @ W.T - y) @ W
     gval = np.zeros((N, T), np.float32)
     for s in range(N):
         gval[s, :cnt_h[s]] = gdense[s, idx_h[s, :cnt_h[s]]]
-    gD, gX = eng.omp_adjoint(Dg, Xg, idx, val, cnt, cu(gval))
+    gD, gX = eng.omp_adjoint(Dg, Xg, idx, val, cnt, cu(gval), Lfac=Lfac)
+    gD2, gX2 = eng.omp_adjoint(Dg, Xg, idx, val, cnt, cu(gval), Lfac=Lfac)
+    assert torch.equal(gD, gD2) and torch.equal(gX, gX2)
     gD, gX = host(gD), host(gX)
     X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    if factored:
+        # the stored factor is the Cholesky factor of G_II in slot order
+        Lh = host(Lfac)
+        for s in range(0, N, 37):
+            m = cnt_h[s]
+            L = np.zeros((T, T))
+            for c in range(T):
+                for r in range(c, T):
+                    L[r, c] = Lh[s, c * T - c * (c - 1) // 2 + (r - c)]
+            Dm = D64[idx_h[s, :m]]
+            np.testing.assert_allclose(L[:m, :m] @ L[:m, :m].T, Dm @ Dm.T, atol=2e-6)
+            assert np.all(L[m:] == 0) and np.all(L[:, m:] == 0)
     val64 = o64.ls_fixed(D64, X64, idx_h, cnt_h)
     gD_ref, gX_ref = o64.ls_adjoint(D64, X64, idx_h, cnt_h, val64, gval.astype(np.float64))
     scaleD, scaleX = np.abs(gD_ref).max(), np.abs(gX_ref).max()
@@ -516,25 +584,59 @@ def test_omp_adjoint_parity_and_finite_differences(o64):
     assert np.abs(gD - gD_ref).max() <= 1e-4 * scaleD
     assert np.abs(gX - gX_ref).max() <= 1e-4 * scaleX
 
+    # the same loss with gamma re-solved on the fixed support, gval held
+    # fixed as the upstream gradient: L(D, X) = <gval, gamma_I(D, X)>
     def loss(Dm, Xm):
         v = o64.ls_fixed(Dm, Xm, idx_h, cnt_h)
-        Gm = H.dense_gamma(idx_h, v, cnt_h, K)
-        return float(np.sum((Gm @ W.T - y) ** 2))
+        return float(np.sum(v * gval.astype(np.float64)))
 
-    rng = np.random.default_rng(3)
-    h = 1e-6
-    errs = []
-    for _ in range(64):
-        s = int(rng.integers(N))
-        j = int(rng.integers(cnt_h[s]))
-        k, t = int(idx_h[s, j]), int(rng.integers(n))
-        Dp, Dm_ = D64.copy(), D64.copy()
-        Dp[k, t] += h
-        Dm_[k, t] -= h
-        fd = (loss(Dp, X64) - loss(Dm_, X64)) / (2 * h)
-        errs.append(abs(fd - gD[k, t]) / max(abs(fd), 1e-3 * scaleD))
-    say("FD rel err (64 entries of D): max", max(errs))
-    assert max(errs) <= 1e-3
+    (nD, eD), (nX, eX) = _fd_errors(loss, D64, X64, gD, gX, idx_h, cnt_h, np.random.default_rng(3))
+    say(f"FD rel err (64 entries each): norm-wise D {nD:.2e} X {nX:.2e}; per entry D {eD:.2e} X {eX:.2e}")
+    assert nD <= FD_TOL and nX <= FD_TOL and eD <= FD_ENTRY_TOL and eX <= FD_ENTRY_TOL
+    eng.close()
+
+
+@pytest.mark.parametrize("store_factors", [True, False])
+def test_autograd_batch_omp(o64, store_factors):
+    """torch.autograd through ksvd_amd.autograd.batch_omp on an unnormalised
+    dictionary (C4's D_init): forward against the FP64 oracle on the
+    normalised atoms; D.grad and X.grad of ||W Gamma - y||^2 against FP64
+    central differences (64 entries of D, 64 of X) of the same loss with the
+    support held fixed and the normalisation included."""
+    from ksvd_amd.autograd import batch_omp
+    say("autograd batch_omp, store_factors", store_factors)
+    wl, X, D0 = _c4_case(o64, N=384)
+    n, K, N, T = 64, 256, 384, 10
+    eng = Engine(n, K, N, T)
+    D = cu(D0).requires_grad_(True)
+    Xt = cu(X).requires_grad_(True)
+    Wt, yt = cu(wl.W.numpy()), cu(wl.y.numpy())
+    Gam, idx, cnt = batch_omp(D, Xt, eng, store_factors)
+    loss = ((Gam @ Wt.T - yt) ** 2).sum()
+    loss.backward()
+    gD, gX = host(D.grad), host(Xt.grad)
+    idx_h, cnt_h = host(idx), host(cnt)
+    # forward vs the oracle on the normalised atoms
+    D64, X64 = D0.astype(np.float64), X.astype(np.float64)
+    Dn64 = o64.normcols(D64)
+    A064 = o64.correlate(Dn64, X64)
+    ref = o64.batch_omp(A064, o64.gram(Dn64), T, gaps=True)
+    eq, tie, mis, bad = H.support_parity(idx_h, cnt_h, ref["idx"], ref["cnt"], ref["gap"],
+                                         1e-4 * np.abs(A064).max(axis=1))
+    say(f"forward supports: equal {eq}/{N}, ties {tie}, mismatches {mis}")
+    assert mis == 0
+    Gd = H.dense_gamma(ref["idx"], ref["val"], ref["cnt"], K)
+    same = (cnt_h == ref["cnt"]) & np.all(idx_h == ref["idx"], axis=1)
+    np.testing.assert_allclose(host(Gam)[same], Gd[same], atol=1e-5 * np.abs(Gd).max())
+    W64, y64 = wl.W.numpy().astype(np.float64), wl.y.numpy().astype(np.float64)
+
+    def loss_fn(Dm, Xm):
+        v = o64.ls_fixed(o64.normcols(Dm), Xm, idx_h, cnt_h)
+        return float(np.sum((H.dense_gamma(idx_h, v, cnt_h, K) @ W64.T - y64) ** 2))
+
+    (nD, eD), (nX, eX) = _fd_errors(loss_fn, D64, X64, gD, gX, idx_h, cnt_h, np.random.default_rng(5))
+    say(f"autograd FD rel err (64 entries each): norm-wise D {nD:.2e} X {nX:.2e}; per entry D {eD:.2e} X {eX:.2e}")
+    assert nD <= FD_TOL and nX <= FD_TOL and eD <= FD_ENTRY_TOL and eX <= FD_ENTRY_TOL
     eng.close()
 
 
