@@ -89,6 +89,27 @@ def profiled_traffic(kernel_src):
     return float(t.group(1))
 
 
+def tuning_report(eng, t, T):
+    """The autotuner's choice (ksvd_autotune) and its measured times, beside
+    the built-in rules' times for the same work."""
+    def r(x):
+        return round(float(x), 4)
+    TT = 8 if T <= 8 else 16 if T <= 16 else 32 if T <= 32 else 48 if T <= 48 else 64
+    shape = (f"{t['omp_kpl']} atoms/lane x {t['omp_waves']} waves/signal (basis {TT})" if t["omp_kpl"]
+             else "G-streaming wave form")
+    return {"source": {2: "measured", 3: "cache"}.get(t["source"], t["source"]),
+            "batch_omp_shape": shape,
+            "error_mode_kernel": ("G-streaming wave form, persistent work-stealing launch" if t["omp_eps_launch"]
+                                  else "register basis, one workgroup per signal"),
+            "gemm_workgroups_per_cu": t["gemm_wpe"], "du_blocks": t["du_blocks"],
+            "ms": {"batch_omp": r(t["omp_ms"]), "gemm": r(t["gemm_ms"]), "du_sweep": r(t["du_ms"])},
+            "built_in_rule_ms": {"batch_omp": r(t["default_omp_ms"]), "gemm": r(t["default_gemm_ms"]),
+                                 "du_sweep": r(t["default_du_ms"])},
+            "candidates": t.get("log", ""),
+            "note": "Batch-OMP timed on up to 65536 signals and the correlation on up to 131072, both scaled to "
+                    "all N; the dictionary-update sweep over all N signals, launched kernel by kernel"}
+
+
 def host_info():
     """Cores this process may use, the CPU model and whether the reference's
     own CPU path (Matlab/fmriKSVD.m under Octave with KSVD-Box) can run."""
@@ -289,6 +310,7 @@ def other_config(args):
         its = p["iters"]
         eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant="auto")
         D0 = wl.D_init.contiguous()
+        out["tuning"] = tuning_report(eng, eng.autotune(D0, X), T)
         rm = []
 
         def run():
@@ -301,6 +323,7 @@ def other_config(args):
         eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant="auto")
         D = eng.normcols(wl.D_true.contiguous())
         eps2 = wl.eps2.contiguous()
+        out["tuning"] = tuning_report(eng, eng.autotune(D, X, eps2), T)
         G = eng.gram(D)
         res = eng.empty_compact()
 
@@ -309,12 +332,25 @@ def other_config(args):
             eng.BatchOMP_eps(D, X, G, eps2, out=res)
         dt = timed(run)
         cnt = res[2].float()
+        # the error-constrained K-SVD iteration (iterations after the first replayed from one hipGraph)
+        D0 = wl.D_init.contiguous()
+        Dout = torch.empty_like(D0)
+        its = 3
+        eng.ExecuteIterations_eps(X, D0, eps2, 1, D_out=Dout, out=res)  # warm-up
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rm = eng.ExecuteIterations_eps(X, D0, eps2, its, D_out=Dout, out=res)[4]
+        torch.cuda.synchronize()
+        dti = (time.perf_counter() - t1) / its
         out.update(metric="error-constrained Batch-OMP signals/s", value=round(N / dt, 1), unit="signals/s",
                    ms_per_pass=round(dt * 1000.0, 3), gram_form=args.form,
-                   mean_support=round(float(cnt.mean()), 3), max_support=int(cnt.max()))
+                   mean_support=round(float(cnt.mean()), 3), max_support=int(cnt.max()),
+                   ksvd_ms_per_iteration=round(dti * 1000.0, 2), ksvd_iterations=its,
+                   ksvd_rmse=[float(v) for v in rm])
     else:  # C4
         eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant="auto")
         D = eng.normcols(wl.D_init.contiguous())
+        out["tuning"] = tuning_report(eng, eng.autotune(D, X), T)
         G = eng.gram(D)
         res = eng.empty_compact()
         gval = torch.randn((N, T), dtype=torch.float32, device=dev, generator=torch.Generator(device=dev).manual_seed(9))
@@ -350,7 +386,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--form", default=None, choices=["fp32", "bf16", "bf16x3", "bf16x6"],
                     help="GEMM form of G and DtX (default: split-BF16x6 for C2, FP32 for --config C1/C3/C4)")
-    ap.add_argument("--omp", default="qform", choices=["qform", "wave16", "wave", "mfma"],
+    ap.add_argument("--omp", default="qform", choices=["qform", "wave"],
                     help="headline Batch-OMP kernel")
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--ksvd-iters", type=int, default=2)
@@ -361,6 +397,7 @@ def main():
     ap.add_argument("--parity-sample", type=int, default=65_536,
                     help="signals of the headline result checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch shapes (no ksvd_autotune)")
     ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
                     help="only run one device copy of GIB GiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4"],
@@ -410,25 +447,26 @@ def main():
 
     eng = Engine(n, K, N, T, gemm_form=args.form, omp_variant=args.omp)
     D = eng.normcols(D0)
+    tuning = None
+    if not args.no_tune:
+        log("autotune")
+        tuning = tuning_report(eng, eng.autotune(D0, X), T)
+        log(f"tuning {tuning}")
     G = torch.empty((K, K), dtype=torch.float32, device=dev)
     idx = torch.empty((N, T), dtype=torch.int32, device=dev)
     val = torch.empty((N, T), dtype=torch.float32, device=dev)
     cnt = torch.empty((N,), dtype=torch.int32, device=dev)
-    A0 = torch.empty((N, K), dtype=torch.float32, device=dev) if args.omp != "mfma" else None
+    A0 = torch.empty((N, K), dtype=torch.float32, device=dev)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
 
     def step(record):
         if record:
             ev[0].record(stream)
         eng.gram(D, G)
-        if A0 is not None:
-            eng.correlate(D, X, A0)
+        eng.correlate(D, X, A0)
         if record:
             ev[1].record(stream)
-        if A0 is not None:
-            eng.batch_omp_compact(A0, G, (idx, val, cnt))
-        else:
-            eng.BatchOMP(D, X, G, (idx, val, cnt))
+        eng.batch_omp_compact(A0, G, (idx, val, cnt))
         if record:
             ev[2].record(stream)
 
@@ -459,7 +497,7 @@ def main():
     plain_steps = args.steps if args.plain_steps is None else args.plain_steps
     if plain_steps > 0:
         eng_p = Engine(n, K, N, T, gemm_form="fp32", omp_variant="wave")
-        A0p = A0 if A0 is not None else torch.empty((N, K), dtype=torch.float32, device=dev)
+        A0p = A0
         Gp = torch.empty_like(G)
         ip, vp, cp = torch.empty_like(idx), torch.empty_like(val), torch.empty_like(cnt)
 
@@ -504,8 +542,9 @@ def main():
     f_sig = np.array([omp_flops_per_signal(K, t) for t in range(T + 1)])
     flops = float(np.sum(f_sig[c]))
     achieved_tf = flops / (omp_avg / 1000.0) / 1e12
-    kname = {"qform": "omp_qform_kernel<4,32,4,false>", "wave16": "omp_wave16_kernel<16,false>",
-             "wave": "omp_wave_kernel<16,false>", "mfma": "omp_fast_kernel<16>"}[args.omp]
+    tk = eng.get_tuning()
+    kname = (f"omp_qform_kernel<{tk['omp_kpl'] or 4},32,{tk['omp_waves'] or 4},false>" if args.omp == "qform"
+             else "omp_wave_kernel<16,false>")
     roofline = {
         "bound": "fp32_valu",
         "achieved": round(achieved_tf, 3),
@@ -559,6 +598,7 @@ def main():
         "omp_fallbacks": fallbacks,
         "plain_fp32_form": plain,
         "parity_vs_fp64_oracle": parity,
+        "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

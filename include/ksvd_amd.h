@@ -55,18 +55,13 @@ enum {
                           /* six BF16 products down to 2^-16, FP32 acc      */
 };
 
-/* Batch-OMP kernel variant. */
+/* Batch-OMP kernel variant (values 2 and 3 are not used). */
 enum {
     KSVD_OMP_AUTO = 0,   /* KSVD_OMP_QFORM, except T > 32 with K > 1024 (its basis */
                          /* would not fit 8 waves' VGPRs): KSVD_OMP_WAVE          */
     KSVD_OMP_WAVE = 1,   /* one wave64 per signal: alpha = alpha0 - G(:,I) gamma   */
-                         /* from FP32 columns of G in VGPRs, L in LDS             */
-    KSVD_OMP_MFMA = 2,   /* 64-signal tiles: alpha = D^T r on fp16 MFMA with a     */
-                         /* rigorous error bound, exact FP32 re-evaluation of the  */
-                         /* candidates; entry points taking D and X, n <= 256     */
-    KSVD_OMP_WAVE16 = 3, /* wave form streaming an fp16 copy of G, rigorous error  */
-                         /* bound per atom, exact FP32 re-evaluation of the        */
-                         /* candidates within it (selection = FP32 arg-max)        */
+                         /* from FP32 columns of G in VGPRs, L in LDS (the plain  */
+                         /* FP32 form, and the fallback for the largest shapes)  */
     KSVD_OMP_QFORM = 4   /* one workgroup per signal holding Q = G(:,I) L^-T in    */
                          /* VGPRs: alpha -= q_i y_i, one G column per step, no     */
                          /* per-step triangular solves; FP32 throughout            */
@@ -96,6 +91,32 @@ typedef struct ksvd_config {
     uint64_t perm_seed;   /* seed of the atom-order permutation stream          */
 } ksvd_config;
 
+/*
+ * Launch shapes.  The reference fixes its launches (S/OMP.cu:109-111: one
+ * thread per signal in blocks of 512; S/FileManager.cu:53 TPB_rowsX); here
+ * they are per-handle settings, chosen by measurement with ksvd_autotune or
+ * set by hand.  All zero selects the built-in rules.  The Batch-OMP shape and
+ * the GEMM build change no result (each value is computed with the same
+ * operations in the same order); du_blocks changes the order in which the
+ * dictionary update sums its block partials (results are repeatable for a
+ * given setting).
+ */
+typedef struct ksvd_tuning {
+    int32_t omp_kpl;        /* register-basis Batch-OMP: atoms per lane (1, 2, 4, 8; 0: rule) */
+    int32_t omp_waves;      /* register-basis Batch-OMP: waves per signal (1, 2, 4, 8; 0: rule)   */
+    int32_t omp_eps_launch; /* error mode with KSVD_OMP_AUTO: 0 the register-basis form (one     */
+                            /* workgroup per signal, the hardware dispatcher starting the next */
+                            /* as each finishes), 1 the G-streaming wave form's persistent     */
+                            /* launch with work stealing over signals                          */
+    int32_t gemm_wpe;       /* GEMM kernel built for 3 (0 or 3) or 2 workgroups per CU           */
+    int32_t du_blocks;      /* workgroups of the per-atom dictionary-update kernels, 64..1024   */
+                            /* (0: 512)                                                       */
+    int32_t source;         /* 0 defaults, 1 ksvd_set_tuning, 2 measured by ksvd_autotune,      */
+                            /* 3 ksvd_autotune cache hit                                       */
+    double omp_ms, gemm_ms, du_ms;                 /* autotune: the chosen candidates' times */
+    double default_omp_ms, default_gemm_ms, default_du_ms; /* and the built-in rules' times   */
+} ksvd_tuning;
+
 /* Fills the reference defaults (T=6, 30 iterations, mu=0.8, use_thresh=4,
  * max_signals=5000), FP32 form, automatic Batch-OMP variant, pivot_fp64 =
  * 1e-3, perm_seed = 0, use_graph = 1. */
@@ -111,6 +132,29 @@ const char* ksvd_last_error(const ksvd_handle* h);
 const char* ksvd_version(void);
 /* Blocks until the handle's stream is idle. */
 int ksvd_sync(ksvd_handle* h);
+
+/* Current launch shapes of the handle. */
+int ksvd_get_tuning(const ksvd_handle* h, ksvd_tuning* out);
+
+/* Sets the launch shapes (time fields ignored); KSVD_ERR_INVALID for a shape
+ * that is not compiled or does not cover K. */
+int ksvd_set_tuning(ksvd_handle* h, const ksvd_tuning* t);
+
+/* Project autotuner: times the candidate launch shapes on the caller's data
+ * -- the GEMM builds on the correlation of up to 131072 signals, the
+ * register-basis Batch-OMP shapes (and, with eps2, both error-mode launches)
+ * on up to 65536 signals, the dictionary-update grids on a full sweep over
+ * all N signals -- and keeps the fastest of each in the handle.  D is
+ * normalised first (into handle scratch); eps2 may be NULL (sparsity mode).
+ * Results are cached for the process by (n, K, N, T, GEMM form, mode, device)
+ * and, when the environment variable KSVD_TUNE_CACHE names a file, in that
+ * file too; force != 0 measures again.  Synchronises.  Replaces the
+ * reference's fixed launch shapes (S/OMP.cu:109-111, S/FileManager.cu:53). */
+int ksvd_autotune(ksvd_handle* h, const float* D, const float* X, const float* eps2, int force);
+
+/* Text of the last ksvd_autotune's measurements: every candidate and its time
+ * ("cache hit" when the choice came from the cache). */
+const char* ksvd_tuning_log(const ksvd_handle* h);
 
 /* ---- primitives ---------------------------------------------------------- */
 
@@ -152,13 +196,20 @@ int ksvd_batch_omp_from_signals_dense(ksvd_handle* h, const float* D, const floa
 
 /* Error-constrained Batch-OMP (no reference counterpart beyond the comment at
  * S/KSVD.cu:79): per signal, stop when ||r||^2 = ||x||^2 - alpha0_I^T gamma_I
- * (evaluated in FP64) <= eps2[s], or at T atoms (T_max).  xnorm2[s] = ||x_s||^2.
- * Runs as one persistent launch with work stealing over signals. */
-int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const float* xnorm2,
+ * (evaluated in FP64) <= eps2[s], or at T atoms (T_max).  xnorm2[s] =
+ * ||x_s||^2 in FP64.  The register-basis form runs one workgroup per signal:
+ * the uneven support sizes are balanced by the hardware dispatcher, which
+ * starts the next queued workgroup on a CU as soon as one finishes.  The
+ * G-streaming wave form runs one persistent launch (as many workgroups as the
+ * GPU holds) whose waves take the next signal from a device counter until all
+ * are done.  With KSVD_OMP_AUTO the tuning (omp_eps_launch) picks between
+ * them; ksvd_autotune measures both. */
+int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const double* xnorm2,
                        const float* eps2, int32_t* idx, float* val, int32_t* cnt);
 
 /* Error-constrained Batch-OMP from D and X with the configured variant:
- * stop when ||r||^2 <= eps2[s] (FP64 sum) or at T atoms. */
+ * stop when ||r||^2 <= eps2[s] (FP64 sum, ||x_s||^2 formed in FP64 from X) or
+ * at T atoms; launches as ksvd_batch_omp_eps. */
 int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float* X,
                                     const float* G, const float* eps2, int32_t* idx, float* val,
                                     int32_t* cnt);
@@ -179,10 +230,9 @@ int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float*
 int ksvd_batch_omp_from_signals_noalloc(ksvd_handle* h, const float* D, const float* X, const float* G,
                                         int N, int32_t* idx, float* val, int32_t* cnt);
 
-/* Diagnostics of the last Batch-OMP call (synchronises): candidates
- * re-evaluated in FP32 and steps that fell back to the full FP32 update
- * (compressed-G form), fallbacks of the MFMA form, or pivots recomputed in
- * FP64 (register-basis form, candidates 0).  NULL outputs allowed. */
+/* Diagnostics of the last Batch-OMP call (synchronises): *fallbacks = the
+ * Cholesky pivots recomputed in FP64 (register-basis form); *candidates is
+ * always 0 (kept for the layout of earlier versions).  NULL outputs allowed. */
 int ksvd_omp_stats(ksvd_handle* h, int64_t* candidates, int64_t* fallbacks);
 
 /* ---- dictionary update / error / clear ------------------------------------ */
@@ -223,6 +273,14 @@ int ksvd_clear_dict(ksvd_handle* h, const float* X, float* D, const int32_t* idx
 int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, float* D_out,
                             int32_t* idx, float* val, int32_t* cnt, int iterations,
                             double* rmse_host);
+
+/* The same loop with error-constrained Batch-OMP (the comment at S/KSVD.cu:79:
+ * sparsecode with a threshold instead of a sparsity): each signal stops when
+ * ||x_s - D gamma_s||^2 <= eps2[s] (FP64 test) or at T atoms (T_max).
+ * Iterations after the first are replayed from one captured hipGraph. */
+int ksvd_execute_iterations_eps(ksvd_handle* h, const float* X, const float* D_in, float* D_out,
+                                const float* eps2, int32_t* idx, float* val, int32_t* cnt,
+                                int iterations, double* rmse_host);
 
 /* Host-only (no device work): the atom-order permutations ksvd_execute_iterations
  * uses -- iterations x K values, row it = the running buffer after it+1

@@ -12,8 +12,8 @@ LIB_PATH = os.path.join(HERE, "libksvd_amd.so")
 
 FORM_FP32, FORM_BF16, FORM_BF16X3, FORM_BF16X6 = 0, 1, 2, 3
 FORMS = {"fp32": FORM_FP32, "bf16": FORM_BF16, "bf16x3": FORM_BF16X3, "bf16x6": FORM_BF16X6}
-OMP_AUTO, OMP_WAVE, OMP_MFMA, OMP_WAVE16, OMP_QFORM = 0, 1, 2, 3, 4
-OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "mfma": OMP_MFMA, "wave16": OMP_WAVE16, "qform": OMP_QFORM}
+OMP_AUTO, OMP_WAVE, OMP_QFORM = 0, 1, 4
+OMP_VARIANTS = {"auto": OMP_AUTO, "wave": OMP_WAVE, "qform": OMP_QFORM}
 
 KSVD_OK = 0
 STATUS = {0: "KSVD_OK", -1: "KSVD_ERR_INVALID", -2: "KSVD_ERR_LIMIT", -3: "KSVD_ERR_HIP", -4: "KSVD_ERR_ALLOC",
@@ -28,7 +28,8 @@ EXPORTS = [
     "ksvd_dictionary_update", "ksvd_dictionary_update_dense", "ksvd_error", "ksvd_clear_dict",
     "ksvd_execute_iterations", "ksvd_perm_sequence", "ksvd_omp_adjoint", "ksvd_mat_read", "ksvd_mat_write",
     "ksvd_batch_omp_compact_factors", "ksvd_omp_adjoint_factored", "ksvd_normcols_adjoint", "ksvd_compact_to_dense",
-    "ksvd_compact_gather",
+    "ksvd_compact_gather", "ksvd_execute_iterations_eps", "ksvd_get_tuning", "ksvd_set_tuning", "ksvd_autotune",
+    "ksvd_tuning_log",
     "ksvd_mat_last_error",
 ]
 
@@ -41,6 +42,19 @@ class Config(C.Structure):
         ("gemm_form", C.c_int32), ("omp_variant", C.c_int32), ("use_graph", C.c_int32),
         ("mu_thresh", C.c_double), ("pivot_fp64", C.c_double), ("perm_seed", C.c_uint64),
     ]
+
+
+class Tuning(C.Structure):
+    """Mirror of struct ksvd_tuning."""
+    _fields_ = [
+        ("omp_kpl", C.c_int32), ("omp_waves", C.c_int32), ("omp_eps_launch", C.c_int32), ("gemm_wpe", C.c_int32),
+        ("du_blocks", C.c_int32), ("source", C.c_int32),
+        ("omp_ms", C.c_double), ("gemm_ms", C.c_double), ("du_ms", C.c_double),
+        ("default_omp_ms", C.c_double), ("default_gemm_ms", C.c_double), ("default_du_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class KsvdError(RuntimeError):
@@ -90,6 +104,11 @@ def load() -> C.CDLL:
     sig("ksvd_error", I, H, P, P, P, P, P, P, P)
     sig("ksvd_clear_dict", I, H, P, P, P, P, P, P)
     sig("ksvd_execute_iterations", I, H, P, P, P, P, P, P, I, P)
+    sig("ksvd_execute_iterations_eps", I, H, P, P, P, P, P, P, P, I, P)
+    sig("ksvd_get_tuning", I, H, C.POINTER(Tuning))
+    sig("ksvd_set_tuning", I, H, C.POINTER(Tuning))
+    sig("ksvd_autotune", I, H, P, P, P, I)
+    sig("ksvd_tuning_log", C.c_char_p, H)
     sig("ksvd_perm_sequence", I, C.c_uint64, I, I, P)
     sig("ksvd_omp_adjoint", I, H, P, P, P, P, P, P, P, P)
     sig("ksvd_batch_omp_compact_factors", I, H, P, P, P, P, P, P)

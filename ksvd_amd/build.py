@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libksvd_amd.so")
 SOURCES = ["omp_qform_t64.hip", "omp_qform_t48.hip", "omp_qform_t32.hip", "omp_qform_t16.hip", "omp_qform_t8.hip",
-           "gemm.hip", "omp.hip", "omp_qform.hip", "omp_fast.hip", "dict_update.hip", "adjoint.hip", "capi.cpp",
+           "gemm.hip", "omp.hip", "omp_qform.hip", "dict_update.hip", "adjoint.hip", "capi.cpp",
            "matio.cpp"]
 MAIN = os.path.join(HERE, "ksvd_main")  # CLI driver (ksvd_main.cpp) linked against the library
 

@@ -7,9 +7,12 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/ksvd_amd.h"
@@ -37,15 +40,16 @@ struct ksvd_handle {
     int rmse_cap = 0;
     uint32_t* perm_all = nullptr;
     int perm_cap = 0;
-    void* fast = nullptr;      // MFMA-form Batch-OMP scratch
-    int fast_grid = 0;
-    float* xnorm2 = nullptr;   // N (wave-form error mode from signals)
+    double* xnorm2 = nullptr;  // N: ||x_s||^2 in FP64 (error mode from signals)
     float* adj_z = nullptr;    // T x N (adjoint: z per support slot)
-    _Float16* gh = nullptr;    // compressed-G form: fp16 G
-    float* gbound = nullptr;   // compressed-G form: per-atom bounds
     unsigned long long* omp_stats = nullptr;  // [0] candidates, [1] fallbacks
     int last_variant = 0;
     bool last_noalloc = false;      // the last Batch-OMP call kept no statistics
+    ksvd_tuning tune{};             // launch shapes (all zero: the built-in rules)
+    std::string tune_log;           // the last autotune's candidates and their times
+    float* tune_D = nullptr;        // autotuner scratch: normalised D, a working copy, coefficients
+    float* tune_D2 = nullptr;
+    float* tune_val = nullptr;
     hipStream_t gstream = nullptr;  // private non-blocking stream for graph capture/replay
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
@@ -82,9 +86,9 @@ hipError_t scratch_ptr(float** out) {
 }
 }  // namespace
 
-// ||x_s||^2 per signal (one wave per signal).
+// ||x_s||^2 per signal in FP64 (one wave per signal).
 __global__ __launch_bounds__(256) void sumsq_cols_kernel(const float* __restrict__ X, int n, int N,
-                                                         float* __restrict__ out) {
+                                                         double* __restrict__ out) {
     const long s = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (s >= N) return;
     const int lane = threadIdx.x & 63;
@@ -94,7 +98,7 @@ __global__ __launch_bounds__(256) void sumsq_cols_kernel(const float* __restrict
         acc += v * v;
     }
     acc = ksvd::wave_sum(acc);
-    if (lane == 0) out[s] = (float)acc;
+    if (lane == 0) out[s] = acc;
 }
 
 namespace {
@@ -144,7 +148,8 @@ int check_cfg(ksvd_handle* h, const ksvd_config* c) {
     if (c->n > 4096) return fail(h, KSVD_ERR_LIMIT, "n=%d exceeds 4096", c->n);
     if ((long)c->N * c->T > 0x7fffffffL) return fail(h, KSVD_ERR_LIMIT, "N*T exceeds 2^31");
     if (c->gemm_form < 0 || c->gemm_form > 3) return fail(h, KSVD_ERR_INVALID, "gemm_form=%d", c->gemm_form);
-    if (c->omp_variant < 0 || c->omp_variant > 4) return fail(h, KSVD_ERR_INVALID, "omp_variant=%d", c->omp_variant);
+    if (c->omp_variant != KSVD_OMP_AUTO && c->omp_variant != KSVD_OMP_WAVE && c->omp_variant != KSVD_OMP_QFORM)
+        return fail(h, KSVD_ERR_INVALID, "omp_variant=%d (KSVD_OMP_AUTO, _WAVE or _QFORM)", c->omp_variant);
     if (c->omp_variant == KSVD_OMP_QFORM && ksvd::qform_shape(c->K, c->T).kpl == 0)
         return fail(h, KSVD_ERR_LIMIT, "register-basis Batch-OMP needs T <= 32 when K > 1024 (K=%d T=%d)", c->K, c->T);
     if (c->mu_thresh < 0) return fail(h, KSVD_ERR_INVALID, "mu_thresh must be >= 0");
@@ -173,8 +178,9 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.list_pos, (size_t)c.N * c.T, "list_pos"));
     TRY(alloc(h, &w.list_rec, (size_t)c.N * c.T, "list_rec"));
     TRY(alloc(h, &w.R, (size_t)c.n * c.N, "R"));
-    TRY(alloc(h, &w.apart, (size_t)ksvd::kDuBlocks * ((c.n + 3) & ~3), "apart"));
-    TRY(alloc(h, &w.bpart, (size_t)2 * ksvd::kDuBlocks, "bpart"));
+    w.blocks = h->tune.du_blocks > 0 ? h->tune.du_blocks : ksvd::kDuBlocks;
+    TRY(alloc(h, &w.apart, (size_t)ksvd::kDuBlocksMax * ((c.n + 3) & ~3), "apart"));
+    TRY(alloc(h, &w.bpart, (size_t)2 * ksvd::kDuBlocksMax, "bpart"));
     TRY(alloc(h, &w.dnew, (size_t)2 * c.n + 32, "dnew"));
     TRY(alloc(h, &w.scal, 8, "scal"));
     TRY(alloc(h, &w.tickets, 4, "tickets"));
@@ -225,46 +231,50 @@ ksvd::OmpArgs omp_args(ksvd_handle* h, const float* DtX, const float* G) {
     a.N = h->cfg.N;
     a.T = h->cfg.T;
     a.pivot_fp64 = (float)h->cfg.pivot_fp64;
+    a.q_kpl = h->tune.omp_kpl;
+    a.q_waves = h->tune.omp_waves;
     return a;
 }
 
-bool use_fast(const ksvd_handle* h) { return h->cfg.omp_variant == KSVD_OMP_MFMA && h->cfg.n <= 256; }
-
-bool use_qform(const ksvd_handle* h) {
+bool use_qform(const ksvd_handle* h, bool eps = false) {
     const int v = h->cfg.omp_variant;
+    // error mode, KSVD_OMP_AUTO: the tuning may pick the wave form's persistent launch
+    if (eps && v == KSVD_OMP_AUTO && h->tune.omp_eps_launch == 1) return false;
     return (v == KSVD_OMP_QFORM || v == KSVD_OMP_AUTO) && ksvd::qform_shape(h->cfg.K, h->cfg.T).kpl != 0;
-}
-
-bool use_g16(const ksvd_handle* h) {
-    return h->cfg.omp_variant == KSVD_OMP_WAVE16 && ksvd::g16_supported(h->cfg.K);
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
-// Fills the wave-form arguments' compressed-G fields when that form is in use.
-int prepare_wave(ksvd_handle* h, const float* G, ksvd::OmpArgs& a) {
-    const ksvd_config& c = h->cfg;
+// Chooses the Batch-OMP kernel (register basis, else the G-streaming wave
+// form) and arms its statistics counters.
+int prepare_wave(ksvd_handle* h, const float* G, ksvd::OmpArgs& a, bool eps = false) {
+    (void)G;
     h->last_noalloc = false;
     TRY(alloc(h, &h->omp_stats, 2, "omp stats"));
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, h->stream, h->omp_stats, 2);
     a.stats = h->omp_stats;
-    if (use_qform(h)) {
-        h->last_variant = KSVD_OMP_QFORM;
-        return KSVD_OK;
-    }
-    h->last_variant = KSVD_OMP_WAVE;
-    if (!use_g16(h)) return KSVD_OK;
-    const int Kpad = ksvd::g16_kpad(c.K);
-    TRY(alloc(h, &h->gh, (size_t)c.K * Kpad, "Gh"));
-    TRY(alloc(h, &h->gbound, (size_t)2 * c.K, "gbound"));
-    HIPTRY(h, ksvd::launch_g16_prep(G, c.K, h->gh, h->gbound, h->stream), "g16 prep");
-    a.Gh = h->gh;
-    a.gbound = h->gbound;
-    a.Kpad = Kpad;
-    h->last_variant = KSVD_OMP_WAVE16;
+    h->last_variant = use_qform(h, eps) ? KSVD_OMP_QFORM : KSVD_OMP_WAVE;
     return KSVD_OK;
+}
+
+// ||x_s||^2 of every signal in FP64 into h->xnorm2.
+int compute_xnorm2(ksvd_handle* h, const float* X) {
+    const ksvd_config& c = h->cfg;
+    TRY(alloc(h, &h->xnorm2, c.N, "xnorm2"));
+    hipLaunchKernelGGL(sumsq_cols_kernel, dim3((unsigned)((c.N + 3) / 4)), dim3(256), 0, h->stream, X, c.n, c.N,
+                       h->xnorm2);
+    return hip_check(h, hipGetLastError(), "xnorm2");
+}
+
+// Error mode: the per-signal stop data, and the work-stealing counter the
+// G-streaming wave form's persistent launch draws signals from (the
+// register-basis form runs one workgroup per signal).
+void set_eps(ksvd_handle* h, ksvd::OmpArgs& a, const double* xnorm2, const float* eps2) {
+    a.xnorm2 = xnorm2;
+    a.eps2 = eps2;
+    a.work_counter = h->counter;
 }
 
 // Launches the wave-family kernel that prepare_wave() chose.
@@ -273,42 +283,258 @@ hipError_t launch_wave_family(const ksvd_handle* h, const ksvd::OmpArgs& a) {
     return ksvd::launch_batch_omp(a, h->stream);
 }
 
-// OMP::BatchOMP from D and X with the configured variant (compact output).
+// OMP::BatchOMP from D and X with the configured variant (compact output);
+// error mode when eps2 != NULL, with ||x_s||^2 already in h->xnorm2.
 int omp_from_signals(ksvd_handle* h, const float* D, const float* X, const float* G, const float* eps2,
                      int32_t* idx, float* val, int32_t* cnt) {
     const ksvd_config& c = h->cfg;
-    if (use_fast(h)) {
-        if (!h->fast) {
-            h->fast_grid = ksvd::fast_grid();
-            void* p = nullptr;
-            TRY(alloc(h, reinterpret_cast<char**>(&p), ksvd::fast_scratch_bytes(c.n, c.K, c.T, h->fast_grid),
-                      "fast scratch"));
-            h->fast = p;
-        }
-        h->last_variant = KSVD_OMP_MFMA;
-        h->last_noalloc = false;
-        return hip_check(h,
-                         ksvd::launch_batch_omp_fast(X, D, G, c.n, c.K, c.N, c.T, eps2, idx, val, cnt,
-                                                     (float)c.pivot_fp64, h->fast, h->fast_grid, h->counter,
-                                                     h->stream),
-                         "batch_omp (mfma)");
-    }
     TRY(alloc(h, &h->dtx, (size_t)c.K * c.N, "DtX"));
-    HIPTRY(h, ksvd::launch_at_times_b(D, c.n, c.K, X, c.N, h->dtx, c.gemm_form, h->stream), "correlate");
+    HIPTRY(h, ksvd::launch_at_times_b(D, c.n, c.K, X, c.N, h->dtx, c.gemm_form, h->stream, h->tune.gemm_wpe), "correlate");
     ksvd::OmpArgs a = omp_args(h, h->dtx, G);
     a.idx = idx;
     a.val = val;
     a.cnt = cnt;
-    TRY(prepare_wave(h, G, a));
-    if (eps2) {
-        TRY(alloc(h, &h->xnorm2, c.N, "xnorm2"));
-        hipLaunchKernelGGL(sumsq_cols_kernel, dim3((unsigned)((c.N + 3) / 4)), dim3(256), 0, h->stream, X, c.n, c.N,
-                           h->xnorm2);
-        a.xnorm2 = h->xnorm2;
-        a.eps2 = eps2;
-        a.work_counter = h->counter;
-    }
+    TRY(prepare_wave(h, G, a, eps2 != nullptr));
+    if (eps2) set_eps(h, a, h->xnorm2, eps2);
     return hip_check(h, launch_wave_family(h, a), "batch_omp (wave)");
+}
+
+// ---- autotuner ---------------------------------------------------------------
+struct TuneKey {
+    int n, K, N, T, form, eps;
+    std::string arch;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(n, K, N, T, form, eps, arch) < std::tie(o.n, o.K, o.N, o.T, o.form, o.eps, o.arch);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, ksvd_tuning> g_tune_cache;
+bool g_tune_file_read = false;
+
+// The cache file named by KSVD_TUNE_CACHE: one line per tuning,
+//   ksvd_tune1 n K N T form eps arch kpl waves eps_launch wpe du_blocks omp gemm du domp dgemm ddu
+void tune_file_read_locked() {
+    if (g_tune_file_read) return;
+    g_tune_file_read = true;
+    const char* path = std::getenv("KSVD_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char arch[128];
+    TuneKey k;
+    ksvd_tuning t{};
+    while (std::fscanf(f, " ksvd_tune1 %d %d %d %d %d %d %127s %d %d %d %d %d %lf %lf %lf %lf %lf %lf", &k.n, &k.K,
+                       &k.N, &k.T, &k.form, &k.eps, arch, &t.omp_kpl, &t.omp_waves, &t.omp_eps_launch, &t.gemm_wpe,
+                       &t.du_blocks, &t.omp_ms, &t.gemm_ms, &t.du_ms, &t.default_omp_ms, &t.default_gemm_ms,
+                       &t.default_du_ms) == 18) {
+        k.arch = arch;
+        t.source = 2;
+        g_tune_cache[k] = t;
+    }
+    std::fclose(f);
+}
+
+void tune_file_append_locked(const TuneKey& k, const ksvd_tuning& t) {
+    const char* path = std::getenv("KSVD_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "ksvd_tune1 %d %d %d %d %d %d %s %d %d %d %d %d %.6f %.6f %.6f %.6f %.6f %.6f\n", k.n, k.K, k.N, k.T,
+                 k.form, k.eps, k.arch.c_str(), t.omp_kpl, t.omp_waves, t.omp_eps_launch, t.gemm_wpe, t.du_blocks,
+                 t.omp_ms, t.gemm_ms, t.du_ms, t.default_omp_ms, t.default_gemm_ms, t.default_du_ms);
+    std::fclose(f);
+}
+
+// Mean time of one call of `fn` (enqueued on h->stream) over `reps` calls
+// after one warm-up call, in ms.
+template <class F>
+int time_ms(ksvd_handle* h, int reps, double* ms, F&& fn) {
+    hipEvent_t e0, e1;
+    HIPTRY(h, hipEventCreate(&e0), "event");
+    HIPTRY(h, hipEventCreate(&e1), "event");
+    int rc = fn();
+    if (rc == KSVD_OK) rc = hip_check(h, hipEventRecord(e0, h->stream), "record");
+    for (int r = 0; r < reps && rc == KSVD_OK; ++r) rc = fn();
+    if (rc == KSVD_OK) rc = hip_check(h, hipEventRecord(e1, h->stream), "record");
+    if (rc == KSVD_OK) rc = hip_check(h, hipEventSynchronize(e1), "sync");
+    float t = 0.f;
+    if (rc == KSVD_OK) rc = hip_check(h, hipEventElapsedTime(&t, e0, e1), "elapsed");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = (double)t / reps;
+    return rc;
+}
+
+int autotune(ksvd_handle* h, const float* D, const float* X, const float* eps2, bool force) {
+    const ksvd_config& c = h->cfg;
+    TuneKey key{c.n, c.K, c.N, c.T, c.gemm_form, eps2 ? 1 : 0, ""};
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        HIPTRY(h, hipGetDevice(&dev), "device");
+        HIPTRY(h, hipGetDeviceProperties(&prop, dev), "device properties");
+        key.arch = prop.gcnArchName;
+        for (char& ch : key.arch)
+            if (ch == ' ') ch = '_';
+    }
+    if (!force) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_file_read_locked();
+        const auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            h->tune = it->second;
+            h->tune.source = 3;
+            h->tune_log = "cache hit";
+            if (h->du_ready) h->du.blocks = h->tune.du_blocks > 0 ? h->tune.du_blocks : ksvd::kDuBlocks;
+            return KSVD_OK;
+        }
+    }
+    const long N = c.N;
+    TRY(alloc(h, &h->tune_D, (size_t)c.n * c.K, "tune D"));
+    TRY(alloc(h, &h->tune_D2, (size_t)c.n * c.K, "tune D2"));
+    TRY(alloc(h, &h->tune_val, (size_t)N * c.T, "tune val"));
+    TRY(alloc(h, &h->gram, (size_t)c.K * c.K, "G"));
+    TRY(alloc(h, &h->dtx, (size_t)c.K * N, "DtX"));
+    TRY(alloc(h, &h->cidx, (size_t)N * c.T, "cidx"));
+    TRY(alloc(h, &h->cval, (size_t)N * c.T, "cval"));
+    TRY(alloc(h, &h->ccnt, N, "ccnt"));
+    TRY(ensure_du(h));
+    if (eps2) TRY(compute_xnorm2(h, X));
+    HIPTRY(h, ksvd::launch_normcols(D, h->tune_D, c.n, c.K, h->stream), "normcols");
+    ksvd_tuning best{};
+    std::string log;
+    auto note = [&](const char* fmt, ...) {
+        char buf[256];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        log += buf;
+    };
+    // 1. GEMM builds, on the correlation of up to 131072 signals (times scaled to N)
+    const long Ng = N < 131072 ? N : 131072;
+    double bestg = 0.0;
+    for (int wpe : {3, 2}) {
+        double ms = 0.0;
+        TRY(time_ms(h, 3, &ms, [&] {
+            return hip_check(h, ksvd::launch_at_times_b(h->tune_D, c.n, c.K, X, Ng, h->dtx, c.gemm_form, h->stream, wpe),
+                             "correlate");
+        }));
+        ms *= (double)N / (double)Ng;
+        note("gemm wpe=%d %.4f ms; ", wpe, ms);
+        if (wpe == 3) best.default_gemm_ms = ms;
+        if (wpe == 3 || ms < bestg) { bestg = ms; best.gemm_wpe = wpe; }
+    }
+    best.gemm_ms = bestg;
+    HIPTRY(h, ksvd::launch_at_times_b(h->tune_D, c.n, c.K, h->tune_D, c.K, h->gram, c.gemm_form, h->stream,
+                                      best.gemm_wpe), "gram");
+    // 2. Batch-OMP shapes (and error-mode launches), on up to 65536 signals
+    const int No = N < 65536 ? (int)N : 65536;
+    HIPTRY(h, ksvd::launch_at_times_b(h->tune_D, c.n, c.K, X, No, h->dtx, c.gemm_form, h->stream, best.gemm_wpe),
+           "correlate");
+    auto omp_run = [&](int kpl, int waves, int eps_launch, int nsig) {
+        ksvd::OmpArgs a = omp_args(h, h->dtx, h->gram);
+        a.N = nsig;
+        a.idx = h->cidx;
+        a.val = h->cval;
+        a.cnt = h->ccnt;
+        a.q_kpl = kpl;
+        a.q_waves = waves;
+        const bool qf = use_qform(h) && eps_launch == 0;
+        if (eps2) {
+            a.xnorm2 = h->xnorm2;
+            a.eps2 = eps2;
+            a.work_counter = h->counter;
+        }
+        return hip_check(h, qf ? ksvd::launch_batch_omp_qform(a, h->stream) : ksvd::launch_batch_omp(a, h->stream),
+                         "batch_omp (autotune)");
+    };
+    {
+        ksvd::QformShape cand[8];
+        const int nc = use_qform(h) ? ksvd::qform_candidates(c.K, c.T, cand, 8) : 0;
+        const ksvd::QformShape def = ksvd::qform_shape(c.K, c.T);
+        double besto = 0.0;
+        bool first = true;
+        // the register-basis shapes, then (error mode, automatic variant, or
+        // no register basis) the G-streaming wave form with its persistent launch
+        for (int i = 0; i <= nc; ++i) {
+            const bool wave = i == nc;
+            if (wave && nc > 0 && !(eps2 && c.omp_variant == KSVD_OMP_AUTO)) break;
+            double ms = 0.0;
+            TRY(time_ms(h, 2, &ms, [&] {
+                return wave ? omp_run(0, 0, 1, No) : omp_run(cand[i].kpl, cand[i].waves, 0, No);
+            }));
+            ms *= (double)N / (double)No;
+            if (wave)
+                note("batch_omp wave form%s %.4f ms; ", eps2 ? " (persistent)" : "", ms);
+            else
+                note("batch_omp kpl=%d waves=%d %.4f ms; ", cand[i].kpl, cand[i].waves, ms);
+            const bool is_def = wave ? nc == 0 : (cand[i].kpl == def.kpl && cand[i].waves == def.waves);
+            if (is_def) best.default_omp_ms = ms;
+            if (first || ms < besto) {
+                besto = ms;
+                best.omp_kpl = wave ? 0 : cand[i].kpl;
+                best.omp_waves = wave ? 0 : cand[i].waves;
+                best.omp_eps_launch = (wave && nc > 0) ? 1 : 0;
+                first = false;
+            }
+        }
+        best.omp_ms = besto;
+    }
+    // 3. dictionary-update grids, on a full sweep over all N signals coded
+    //    with the chosen shapes (identity atom order)
+    HIPTRY(h, ksvd::launch_at_times_b(h->tune_D, c.n, c.K, X, N, h->dtx, c.gemm_form, h->stream, best.gemm_wpe),
+           "correlate");
+    TRY(omp_run(best.omp_kpl, best.omp_waves, best.omp_eps_launch, (int)N));
+    {
+        std::vector<uint32_t> ident(c.K);
+        for (int k = 0; k < c.K; ++k) ident[k] = (uint32_t)k;
+        HIPTRY(h, hipMemcpyAsync(h->du.perm_dev, ident.data(), sizeof(uint32_t) * c.K, hipMemcpyHostToDevice, h->stream),
+               "perm upload");
+        HIPTRY(h, hipStreamSynchronize(h->stream), "perm upload sync");
+    }
+    double bestd = 0.0;
+    for (int blocks : {512, 256, 768, 1024}) {
+        h->du.blocks = blocks;
+        double ms = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {  // the first sweep warms up
+            HIPTRY(h, hipMemcpyAsync(h->tune_D2, h->tune_D, sizeof(float) * c.n * c.K, hipMemcpyDeviceToDevice,
+                                     h->stream), "copy D");
+            HIPTRY(h, hipMemcpyAsync(h->tune_val, h->cval, sizeof(float) * N * c.T, hipMemcpyDeviceToDevice, h->stream),
+                   "copy val");
+            HIPTRY(h, ksvd::launch_du_init(h->du, h->stream), "du_init");
+            double one = 0.0;
+            hipEvent_t e0, e1;
+            HIPTRY(h, hipEventCreate(&e0), "event");
+            HIPTRY(h, hipEventCreate(&e1), "event");
+            HIPTRY(h, hipEventRecord(e0, h->stream), "record");
+            HIPTRY(h, ksvd::launch_du_sweep(h->du, X, h->tune_D2, h->cidx, h->tune_val, h->ccnt, h->stream),
+                   "dictionary update (autotune)");
+            HIPTRY(h, hipEventRecord(e1, h->stream), "record");
+            HIPTRY(h, hipEventSynchronize(e1), "sync");
+            float t = 0.f;
+            HIPTRY(h, hipEventElapsedTime(&t, e0, e1), "elapsed");
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            one = t;
+            if (rep > 0 && one < ms) ms = one;
+        }
+        note("du blocks=%d %.4f ms; ", blocks, ms);
+        if (blocks == 512) best.default_du_ms = ms;
+        if (blocks == 512 || ms < bestd) { bestd = ms; best.du_blocks = blocks; }
+    }
+    best.du_ms = bestd;
+    h->du.blocks = best.du_blocks;
+    HIPTRY(h, ksvd::launch_du_init(h->du, h->stream), "du_init");
+    HIPTRY(h, hipStreamSynchronize(h->stream), "sync");
+    best.source = 2;
+    h->tune = best;
+    h->tune_log = log;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_file_read_locked();
+    g_tune_cache[key] = best;
+    tune_file_append_locked(key, best);
+    return KSVD_OK;
 }
 
 }  // namespace
@@ -393,7 +619,7 @@ int ksvd_at_times_b(ksvd_handle* h, const float* A, int rows, int colsA, const f
     if (!A || !B || !C) return fail(h, KSVD_ERR_NULL, "at_times_b: null pointer");
     if (rows < 1 || colsA < 1 || colsB < 1)
         return fail(h, KSVD_ERR_INVALID, "at_times_b: bad shape rows=%d colsA=%d colsB=%d", rows, colsA, colsB);
-    return hip_check(h, ksvd::launch_at_times_b(A, rows, colsA, B, colsB, C, h->cfg.gemm_form, h->stream),
+    return hip_check(h, ksvd::launch_at_times_b(A, rows, colsA, B, colsB, C, h->cfg.gemm_form, h->stream, h->tune.gemm_wpe),
                      "at_times_b");
 }
 
@@ -450,8 +676,6 @@ int ksvd_batch_omp_from_signals(ksvd_handle* h, const float* D, const float* X, 
                                 float* val, int32_t* cnt) {
     if (!h) return KSVD_ERR_NULL;
     if (!D || !X || !G || !idx || !val || !cnt) return fail(h, KSVD_ERR_NULL, "batch_omp_from_signals: null pointer");
-    if (h->cfg.omp_variant == KSVD_OMP_MFMA && h->cfg.n > 256)
-        return fail(h, KSVD_ERR_LIMIT, "MFMA Batch-OMP needs n <= 256 (n=%d)", h->cfg.n);
     return omp_from_signals(h, D, X, G, nullptr, idx, val, cnt);
 }
 
@@ -475,8 +699,7 @@ int ksvd_batch_omp_eps_from_signals(ksvd_handle* h, const float* D, const float*
     if (!h) return KSVD_ERR_NULL;
     if (!D || !X || !G || !eps2 || !idx || !val || !cnt)
         return fail(h, KSVD_ERR_NULL, "batch_omp_eps_from_signals: null pointer");
-    if (h->cfg.omp_variant == KSVD_OMP_MFMA && h->cfg.n > 256)
-        return fail(h, KSVD_ERR_LIMIT, "MFMA Batch-OMP needs n <= 256 (n=%d)", h->cfg.n);
+    TRY(compute_xnorm2(h, X));
     return omp_from_signals(h, D, X, G, eps2, idx, val, cnt);
 }
 
@@ -506,7 +729,7 @@ int ksvd_batch_omp_from_signals_noalloc(ksvd_handle* h, const float* D, const fl
     h->last_noalloc = true;
     for (long s0 = 0; s0 < N; s0 += chunk) {
         const long nc = (N - s0 < chunk) ? N - s0 : chunk;
-        HIPTRY(h, ksvd::launch_at_times_b(D, c.n, c.K, X + s0 * c.n, nc, dtx, c.gemm_form, h->stream),
+        HIPTRY(h, ksvd::launch_at_times_b(D, c.n, c.K, X + s0 * c.n, nc, dtx, c.gemm_form, h->stream, h->tune.gemm_wpe),
                "correlate (noalloc)");
         ksvd::OmpArgs a = omp_args(h, dtx, G);
         a.N = (int)nc;
@@ -522,21 +745,15 @@ int ksvd_batch_omp_from_signals_noalloc(ksvd_handle* h, const float* D, const fl
 int ksvd_omp_stats(ksvd_handle* h, int64_t* candidates, int64_t* fallbacks) {
     if (!h) return KSVD_ERR_NULL;
     unsigned long long st[2] = {0ull, 0ull};
-    int32_t fb = 0;
-    if (h->last_noalloc) {
-        // the allocation-free entry point counts nothing
-    } else if (h->last_variant == KSVD_OMP_MFMA) {
-        HIPTRY(h, hipMemcpyAsync(&fb, h->counter + 1, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream), "download");
-    } else if (h->omp_stats) {
+    if (!h->last_noalloc && h->omp_stats)  // the allocation-free entry point counts nothing
         HIPTRY(h, hipMemcpyAsync(st, h->omp_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream), "download");
-    }
     HIPTRY(h, hipStreamSynchronize(h->stream), "sync");
     if (candidates) *candidates = (int64_t)st[0];
-    if (fallbacks) *fallbacks = h->last_variant == KSVD_OMP_MFMA ? (int64_t)fb : (int64_t)st[1];
+    if (fallbacks) *fallbacks = (int64_t)st[1];
     return KSVD_OK;
 }
 
-int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const float* xnorm2, const float* eps2,
+int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const double* xnorm2, const float* eps2,
                        int32_t* idx, float* val, int32_t* cnt) {
     if (!h) return KSVD_ERR_NULL;
     if (!DtX || !G || !xnorm2 || !eps2 || !idx || !val || !cnt)
@@ -545,10 +762,8 @@ int ksvd_batch_omp_eps(ksvd_handle* h, const float* DtX, const float* G, const f
     a.idx = idx;
     a.val = val;
     a.cnt = cnt;
-    a.xnorm2 = xnorm2;
-    a.eps2 = eps2;
-    a.work_counter = h->counter;
-    TRY(prepare_wave(h, G, a));
+    TRY(prepare_wave(h, G, a, true));
+    set_eps(h, a, xnorm2, eps2);
     return hip_check(h, launch_wave_family(h, a), "batch_omp_eps");
 }
 
@@ -615,14 +830,15 @@ int ksvd_clear_dict(ksvd_handle* h, const float* X, float* D, const int32_t* idx
     return hip_check(h, ksvd::launch_clear_dict(h->du, X, D, idx, val, cnt, err, h->stream), "clear_dict");
 }
 
-int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, float* D_out, int32_t* idx, float* val,
-                            int32_t* cnt, int iterations, double* rmse_host) {
-    if (!h) return KSVD_ERR_NULL;
+// The iteration loop of ksvd_execute_iterations[_eps] (eps2 NULL: sparsity mode).
+static int execute_iterations(ksvd_handle* h, const float* X, const float* D_in, float* D_out, const float* eps2,
+                              int32_t* idx, float* val, int32_t* cnt, int iterations, double* rmse_host) {
     if (!X || !D_in || !D_out || !idx || !val || !cnt)
         return fail(h, KSVD_ERR_NULL, "execute_iterations: null pointer");
     if (iterations < 0) return fail(h, KSVD_ERR_INVALID, "iterations=%d", iterations);
     const ksvd_config& c = h->cfg;
     TRY(ensure_du(h));
+    if (eps2) TRY(compute_xnorm2(h, X));  // X is fixed: once, outside the captured iteration
     TRY(alloc(h, &h->gram, (size_t)c.K * c.K, "G"));
     TRY(alloc(h, &h->errv, c.N, "err"));
     if (iterations > h->rmse_cap || iterations > h->perm_cap) {
@@ -647,8 +863,8 @@ int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, f
     // into du.perm_dev and its RMSE read from du.rmse_dev around it), so the
     // same launch sequence can be captured once and replayed.
     auto enqueue = [&]() -> int {
-        HIPTRY(h, ksvd::launch_at_times_b(D_out, c.n, c.K, D_out, c.K, h->gram, c.gemm_form, h->stream), "gram");
-        TRY(omp_from_signals(h, D_out, X, h->gram, nullptr, idx, val, cnt));
+        HIPTRY(h, ksvd::launch_at_times_b(D_out, c.n, c.K, D_out, c.K, h->gram, c.gemm_form, h->stream, h->tune.gemm_wpe), "gram");
+        TRY(omp_from_signals(h, D_out, X, h->gram, eps2, idx, val, cnt));
         HIPTRY(h, ksvd::launch_du_init(h->du, h->stream), "du_init");
         HIPTRY(h, ksvd::launch_du_sweep(h->du, X, D_out, idx, val, cnt, h->stream), "dictionary_update");
         HIPTRY(h, ksvd::launch_error_from_residual(h->du, h->errv, h->stream), "error");
@@ -713,6 +929,53 @@ int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, f
         HIPTRY(h, hipStreamSynchronize(h->stream), "rmse sync");
     }
     return KSVD_OK;
+}
+
+int ksvd_execute_iterations(ksvd_handle* h, const float* X, const float* D_in, float* D_out, int32_t* idx, float* val,
+                            int32_t* cnt, int iterations, double* rmse_host) {
+    if (!h) return KSVD_ERR_NULL;
+    return execute_iterations(h, X, D_in, D_out, nullptr, idx, val, cnt, iterations, rmse_host);
+}
+
+int ksvd_execute_iterations_eps(ksvd_handle* h, const float* X, const float* D_in, float* D_out, const float* eps2,
+                                int32_t* idx, float* val, int32_t* cnt, int iterations, double* rmse_host) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!eps2) return fail(h, KSVD_ERR_NULL, "execute_iterations_eps: null eps2");
+    return execute_iterations(h, X, D_in, D_out, eps2, idx, val, cnt, iterations, rmse_host);
+}
+
+int ksvd_get_tuning(const ksvd_handle* h, ksvd_tuning* out) {
+    if (!h || !out) return KSVD_ERR_NULL;
+    *out = h->tune;
+    return KSVD_OK;
+}
+
+const char* ksvd_tuning_log(const ksvd_handle* h) { return h ? h->tune_log.c_str() : ""; }
+
+int ksvd_set_tuning(ksvd_handle* h, const ksvd_tuning* t) {
+    if (!h || !t) return KSVD_ERR_NULL;
+    const ksvd_config& c = h->cfg;
+    if ((t->omp_kpl != 0) != (t->omp_waves != 0))
+        return fail(h, KSVD_ERR_INVALID, "set_tuning: omp_kpl and omp_waves are set together");
+    if (t->omp_kpl && !ksvd::qform_shape_with(c.K, c.T, t->omp_kpl, t->omp_waves).kpl)
+        return fail(h, KSVD_ERR_INVALID, "set_tuning: register-basis shape kpl=%d waves=%d is not compiled for "
+                    "K=%d T=%d", t->omp_kpl, t->omp_waves, c.K, c.T);
+    if (t->omp_eps_launch != 0 && t->omp_eps_launch != 1)
+        return fail(h, KSVD_ERR_INVALID, "set_tuning: omp_eps_launch=%d", t->omp_eps_launch);
+    if (t->gemm_wpe != 0 && t->gemm_wpe != 2 && t->gemm_wpe != 3)
+        return fail(h, KSVD_ERR_INVALID, "set_tuning: gemm_wpe=%d (2 or 3)", t->gemm_wpe);
+    if (t->du_blocks != 0 && (t->du_blocks < 64 || t->du_blocks > ksvd::kDuBlocksMax))
+        return fail(h, KSVD_ERR_INVALID, "set_tuning: du_blocks=%d (64..%d)", t->du_blocks, ksvd::kDuBlocksMax);
+    h->tune = *t;
+    h->tune.source = 1;
+    if (h->du_ready) h->du.blocks = t->du_blocks > 0 ? t->du_blocks : ksvd::kDuBlocks;
+    return KSVD_OK;
+}
+
+int ksvd_autotune(ksvd_handle* h, const float* D, const float* X, const float* eps2, int force) {
+    if (!h) return KSVD_ERR_NULL;
+    if (!D || !X) return fail(h, KSVD_ERR_NULL, "autotune: null pointer");
+    return autotune(h, D, X, eps2, force != 0);
 }
 
 int ksvd_perm_sequence(uint64_t seed, int K, int iterations, uint32_t* out) {

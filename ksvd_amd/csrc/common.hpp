@@ -202,8 +202,9 @@ namespace ksvd {
 
 // C (colsA x colsB) = A^T B, column-major, lda = ldb = rows.  form: 0 fp32,
 // 1 bf16, 2 bf16x3, 3 bf16x6.
+// wpe: workgroups per CU the kernel is compiled for, 2 or 3 (0: 3).
 hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B, long colsB, float* C,
-                             int form, hipStream_t st);
+                             int form, hipStream_t st, int wpe = 0);
 hipError_t launch_normcols(const float* Din, float* Dout, int n, int K, hipStream_t st);
 
 struct OmpArgs {
@@ -215,21 +216,17 @@ struct OmpArgs {
     int32_t* idx;       // optional T x N compact
     float* val;         // optional T x N compact
     int32_t* cnt;       // optional N
-    const float* xnorm2;  // error mode (else NULL)
+    const double* xnorm2;  // error mode: ||x_s||^2 in FP64 (else NULL)
     const float* eps2;
     int32_t* work_counter;  // persistent work-stealing counter (error mode)
     float pivot_fp64;
-    const _Float16* Gh;     // compressed-G form: fp16 G, K columns of Kpad (NULL: FP32 form)
-    const float* gbound;    // compressed-G form: per atom (g_k, d_k)
-    int Kpad;
-    unsigned long long* stats;  // compressed-G form: [0] candidates re-evaluated, [1] FP32 fallbacks
+    unsigned long long* stats;  // [1]: pivots recomputed in FP64 (register-basis form); [0] unused
+    int q_kpl, q_waves;         // register-basis shape (0: the default rule)
     float* Lfac;                // optional N x T(T+1)/2: each signal's Cholesky factor L (L L^T = G_II),
                                 // lower triangle packed column by column (register-basis form)
 };
 hipError_t launch_batch_omp(const OmpArgs& a, hipStream_t st);
-int g16_kpad(int K);
-bool g16_supported(int K);
-hipError_t launch_g16_prep(const float* G, int K, _Float16* Gh, float* gbound, hipStream_t st);
+hipError_t launch_zero_counter(int32_t* p, hipStream_t st);  // *p = 0 (a kernel: graph-capturable)
 
 // Batch-OMP with the Gram-domain basis Q = G(:,I) L^-T in registers
 // (omp_qform.hip): one workgroup of `waves` waves per signal, `kpl` atoms per
@@ -238,19 +235,18 @@ struct QformShape {
     int kpl, tt, waves;
 };
 QformShape qform_shape(int K, int T);
+// (kpl, waves) as given when compiled and covering K, else kpl == 0
+QformShape qform_shape_with(int K, int T, int kpl, int waves);
+// the compiled shapes for (K, T): the fewest waves for each kpl; returns the count
+int qform_candidates(int K, int T, QformShape* out, int cap);
 hipError_t launch_batch_omp_qform(const OmpArgs& a, hipStream_t st);
 
-// Batch-OMP with the correlation update on MFMA (omp_fast.hip); n <= 256.
-size_t fast_scratch_bytes(int n, int K, int T, int grid);
-int fast_grid();
-hipError_t launch_batch_omp_fast(const float* X, const float* D, const float* G, int n, int K, int N, int T,
-                                 const float* eps2, int32_t* idx, float* val, int32_t* cnt, float pivot_fp64,
-                                 void* scratch, int grid, int32_t* counters, hipStream_t st);
 hipError_t launch_compact_to_slotmap(const int32_t* idx, const float* val, const int32_t* cnt, int K, int N, int T,
                                      float* sel, float* c, hipStream_t st);
 
 // Dictionary update / error / clear (dict_update.hip)
-constexpr int kDuBlocks = 512;  // workgroups of the per-atom phase kernels (one block partial each)
+constexpr int kDuBlocks = 512;      // default workgroups of the per-atom phase kernels (one block partial each)
+constexpr int kDuBlocksMax = 1024;  // the most the scratch (apart, bpart) is sized for
 struct DuWork {
     int n, K, N, T, max_signals, use_thresh;
     double mu_thresh;
@@ -263,8 +259,9 @@ struct DuWork {
                          // gamma bits, bits of the signal's gamma on the next atom in the sweep order (0 if
                          // unused), 0}
     float* R;            // n x N restricted residuals X - D Gamma
-    float* apart;        // kDuBlocks x (n rounded up to 4) partial a
-    float* bpart;        // kDuBlocks x 2 partial arg-max (special case)
+    int blocks;          // workgroups of the per-atom kernels (<= kDuBlocksMax)
+    float* apart;        // blocks x (n rounded up to 4) partial a
+    float* bpart;        // blocks x 2 partial arg-max (special case)
     float* dnew;         // 2 x n + 2 x 16: a and its partial sums of squares, by sweep-position parity
     float* scal;         // 8 scalars (norm etc.)
     int32_t* tickets;    // 4 last-block-done counters (zero between launches)

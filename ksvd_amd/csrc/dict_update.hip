@@ -25,7 +25,6 @@ namespace ksvd {
 
 namespace {
 
-constexpr int DU_BLOCKS = kDuBlocks;  // grid of the per-atom phase kernels (grid-stride over the atom's list)
 constexpr int RES_BLOCKS = 2048;   // grid of the residual initialisation
 
 __device__ __forceinline__ void wave_lds_fence() {
@@ -951,17 +950,18 @@ void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hip
     // reduction of jj + 1 (two launches per atom)
     hipLaunchKernelGGL(du_sweep_desc_kernel, dim3((w.K + 255) / 256), dim3(256), 0, st, w.perm_dev, w.count, w.ptr,
                        w.K, w.sweep);
-    hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
+    const int nb = w.blocks;  // grid of the per-atom kernels (grid-stride over the atom's list)
+    hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(nb), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
                        w.list_rec, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
                        M, w.apart, w.bpart, abuf, anorm, w.tickets);
-    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.apart, abuf, anorm, w.n, DU_BLOCKS);
+    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.apart, abuf, anorm, w.n, nb);
     for (int jj = 0; jj < w.K; ++jj) {
-        hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(DU_BLOCKS), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
+        hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(nb), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
                            w.ptr, w.list_rec, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
                            w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets, w.sweep);
         if (jj + 1 < w.K)
             hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.apart, abuf, anorm, w.n,
-                               DU_BLOCKS);
+                               nb);
     }
     hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n);
 }
@@ -1261,10 +1261,10 @@ hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* i
     hipLaunchKernelGGL(du_residual_kernel, dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n, w.N, w.T, w.R);
     const size_t lds_a = (size_t)4 * w.n * sizeof(float);
     for (int jj = 0; jj < w.K; ++jj) {
-        hipLaunchKernelGGL(du_phase_a_kernel, dim3(DU_BLOCKS), dim3(256), lds_a, st, jj, w.perm_dev, w.count, w.ptr,
+        hipLaunchKernelGGL(du_phase_a_kernel, dim3(w.blocks), dim3(256), lds_a, st, jj, w.perm_dev, w.count, w.ptr,
                            w.list_pos, val, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T, M,
                            w.apart, w.bpart, w.dnew, w.scal, w.tickets);
-        hipLaunchKernelGGL(du_phase_b_kernel, dim3(DU_BLOCKS), dim3(256), 0, st, jj, w.perm_dev, w.count, w.ptr,
+        hipLaunchKernelGGL(du_phase_b_kernel, dim3(w.blocks), dim3(256), 0, st, jj, w.perm_dev, w.count, w.ptr,
                            w.list_pos, val, D, w.R, w.dnew, w.n, w.T, w.tickets + 1);
     }
     return hipGetLastError();

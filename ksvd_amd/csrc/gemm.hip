@@ -111,12 +111,14 @@ __device__ __forceinline__ void put_tile(float* __restrict__ s, const float4* v)
     }
 }
 
-// Three workgroups per CU: the split form needs 192 registers per lane at
-// the compiler's default and would run two (3.03 ms against 2.70 ms for the
-// n=256, K=1024, N=10^6 correlation, with a few spilled registers); the FP32
-// and BF16 forms fit three either way.
-template <int FORM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
+// Compiled for WPE workgroups per CU (3 by default: the split form needs 192
+// registers per lane at the compiler's default and would run two -- 3.03 ms
+// against 2.70 ms for the n=256, K=1024, N=10^6 correlation, with a few
+// spilled registers; the FP32 and BF16 forms fit three either way; the
+// three-part split's 60 kB of LDS allow two).  WPE changes the register
+// budget only, not the arithmetic: the autotuner may pick 2.
+template <int FORM, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
                                                          const float* __restrict__ B, long colsB,
                                                          float* __restrict__ C, int tilesA) {
     // FP32: tiles in LDS as FP32; BF16 forms: as bf16 hi (and lo)
@@ -259,10 +261,24 @@ __global__ __launch_bounds__(256) void normcols_kernel(const float* __restrict__
     for (int t = lane; t < n; t += 64) Dout[(long)k * n + t] = d[t] * coeff;
 }
 
+template <int WPE>
+hipError_t launch_atb(const float* A, int rows, int colsA, const float* B, long colsB, float* C, int form,
+                      dim3 grid, int tilesA, hipStream_t st) {
+    const dim3 block(256);
+    switch (form) {
+        case 0: hipLaunchKernelGGL((at_times_b_kernel<0, WPE>), grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        case 1: hipLaunchKernelGGL((at_times_b_kernel<1, WPE>), grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        case 2: hipLaunchKernelGGL((at_times_b_kernel<2, WPE>), grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        case 3: hipLaunchKernelGGL((at_times_b_kernel<3, WPE>), grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B, long colsB, float* C, int form,
-                             hipStream_t st) {
+                             hipStream_t st, int wpe) {
     if (rows <= 0 || colsA <= 0 || colsB <= 0) return hipSuccess;
     // tile-relative offsets are 32-bit: 128 columns of A, B and C
     if ((long)rows * BM > 0x7fffffffL || (long)colsA * BM > 0x7fffffffL) return hipErrorInvalidValue;
@@ -270,15 +286,9 @@ hipError_t launch_at_times_b(const float* A, int rows, int colsA, const float* B
     const long tilesB = ((colsB + BM - 1) / BM + 7) / 8 * 8;  // whole groups of 8 (one per XCD)
     const long blocks = tilesB * tilesA;
     if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-    dim3 grid((unsigned)blocks), block(256);
-    switch (form) {
-        case 0: hipLaunchKernelGGL(at_times_b_kernel<0>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
-        case 1: hipLaunchKernelGGL(at_times_b_kernel<1>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
-        case 2: hipLaunchKernelGGL(at_times_b_kernel<2>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
-        case 3: hipLaunchKernelGGL(at_times_b_kernel<3>, grid, block, 0, st, A, rows, colsA, B, colsB, C, tilesA); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const dim3 grid((unsigned)blocks);
+    if (wpe == 2) return launch_atb<2>(A, rows, colsA, B, colsB, C, form, grid, tilesA, st);
+    return launch_atb<3>(A, rows, colsA, B, colsB, C, form, grid, tilesA, st);
 }
 
 hipError_t launch_normcols(const float* Din, float* Dout, int n, int K, hipStream_t st) {

@@ -8,7 +8,7 @@
 //
 //   ksvd_main [--X ./input/X.mat] [--D ./input/D.mat] [--out ./output/temp.mat]
 //             [--dict-out path] [--iters 30] [--T 6] [--mu 0.8] [--use-thresh 4]
-//             [--seed 0] [--omp auto|wave|mfma|wave16|qform] [--form fp32|bf16|bf16x3]
+//             [--seed 0] [--omp auto|wave|qform] [--form fp32|bf16|bf16x3|bf16x6]
 //             [--no-graph]
 #include <hip/hip_runtime.h>
 
@@ -33,8 +33,8 @@ struct Options {
 void usage() {
     std::printf(
         "usage: ksvd_main [--X path] [--D path] [--out path] [--dict-out path] [--iters N] [--T N]\n"
-        "                 [--mu x] [--use-thresh N] [--seed S] [--omp auto|wave|mfma|wave16|qform]\n"
-        "                 [--form fp32|bf16|bf16x3] [--no-graph]\n"
+        "                 [--mu x] [--use-thresh N] [--seed S] [--omp auto|wave|qform]\n"
+        "                 [--form fp32|bf16|bf16x3|bf16x6] [--no-graph]\n"
         "Reads X (n x N) and D (n x K) from MAT-files (variables X and D), runs the K-SVD\n"
         "iterations and writes the final dense Gamma as CUDA_Gamma.\n");
 }
@@ -66,8 +66,6 @@ bool parse(int argc, char** argv, Options& o) {
             const std::string s = v;
             if (s == "auto") o.omp = KSVD_OMP_AUTO;
             else if (s == "wave") o.omp = KSVD_OMP_WAVE;
-            else if (s == "mfma") o.omp = KSVD_OMP_MFMA;
-            else if (s == "wave16") o.omp = KSVD_OMP_WAVE16;
             else if (s == "qform") o.omp = KSVD_OMP_QFORM;
             else { std::fprintf(stderr, "unknown --omp %s\n", v); return false; }
         } else if (a == "--form") {
@@ -76,6 +74,7 @@ bool parse(int argc, char** argv, Options& o) {
             if (s == "fp32") o.form = KSVD_FORM_FP32;
             else if (s == "bf16") o.form = KSVD_FORM_BF16;
             else if (s == "bf16x3") o.form = KSVD_FORM_BF16X3;
+            else if (s == "bf16x6") o.form = KSVD_FORM_BF16X6;
             else { std::fprintf(stderr, "unknown --form %s\n", v); return false; }
         } else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());

@@ -1,6 +1,10 @@
 // omp_qform.hip -- shape choice and dispatch of the register-basis Batch-OMP
 // kernel (omp_qform.hpp); one translation unit per basis size TT
-// (omp_qform_t{8,16,32,64}.hip) holds its instantiations.
+// (omp_qform_t{8,16,32,48,64}.hip) holds its instantiations.  The default
+// shape is a fixed rule (qform_shape); the autotuner picks among
+// qform_candidates by measurement.  Every shape computes each atom's values
+// with the same operations in the same order, so the results do not depend
+// on the shape.
 #include "omp_common.hpp"
 
 namespace ksvd {
@@ -29,9 +33,45 @@ QformShape qform_shape(int K, int T) {
     return sh;
 }
 
+// The (atoms per lane, waves per signal) pairs compiled for each basis size
+// (omp_qform_t*.hip); a shape also needs 64 * kpl * waves >= K.
+static bool qform_compiled(int tt, int kpl, int waves) {
+    if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return false;
+    switch (tt) {
+        case 8:
+        case 16: return kpl == 1 || kpl == 2 || kpl == 4 || kpl == 8;
+        case 32: return kpl == 1 || kpl == 2 || kpl == 4;
+        case 48: return kpl == 1 || kpl == 2;
+        case 64: return kpl == 2 || (kpl == 1 && waves == 1);
+    }
+    return false;
+}
+
+QformShape qform_shape_with(int K, int T, int kpl, int waves) {
+    QformShape sh = qform_shape(K, T);
+    if (!sh.kpl || !qform_compiled(sh.tt, kpl, waves) || 64L * kpl * waves < K) return QformShape{0, 0, 0};
+    sh.kpl = kpl;
+    sh.waves = waves;
+    return sh;
+}
+
+int qform_candidates(int K, int T, QformShape* out, int cap) {
+    const QformShape d = qform_shape(K, T);
+    if (!d.kpl) return 0;
+    int n = 0;
+    for (int kpl = 1; kpl <= 8; kpl *= 2) {
+        // the fewest waves that cover K with this many atoms per lane
+        int W = 1;
+        while (64L * kpl * W < K && W < 8) W *= 2;
+        const QformShape sh = qform_shape_with(K, T, kpl, W);
+        if (sh.kpl && n < cap) out[n++] = sh;
+    }
+    return n;
+}
+
 hipError_t launch_batch_omp_qform(const OmpArgs& a, hipStream_t st) {
     if (a.N <= 0) return hipSuccess;
-    const QformShape sh = qform_shape(a.K, a.T);
+    const QformShape sh = a.q_kpl ? qform_shape_with(a.K, a.T, a.q_kpl, a.q_waves) : qform_shape(a.K, a.T);
     if (!sh.kpl) return hipErrorInvalidValue;
     switch (sh.tt) {
         case 8: return launch_qform_t8(a, sh, st);

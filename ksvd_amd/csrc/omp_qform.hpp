@@ -189,7 +189,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
     Support sp;
     double xn2 = 0.0, e2 = 0.0, ysum = 0.0;
     if constexpr (EPS) {
-        xn2 = (double)a.xnorm2[s];
+        xn2 = a.xnorm2[s];
         e2 = (double)a.eps2[s];
     }
     unsigned n_redo = 0;

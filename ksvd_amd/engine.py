@@ -89,6 +89,30 @@ class Engine:
     def sync(self):
         self._rc(self.lib.ksvd_sync(self.h))
 
+    # -- launch shapes ----------------------------------------------------------------
+    def get_tuning(self) -> dict:
+        t = _lib.Tuning()
+        self._rc(self.lib.ksvd_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Sets launch-shape fields (omp_kpl, omp_waves, omp_eps_launch,
+        gemm_wpe, du_blocks); the others keep their current values."""
+        t = _lib.Tuning()
+        self._rc(self.lib.ksvd_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._rc(self.lib.ksvd_set_tuning(self.h, C.byref(t)))
+        return self.get_tuning()
+
+    def autotune(self, D, X, eps2=None, force=False) -> dict:
+        """ksvd_autotune: measures the candidate launch shapes on D and X and
+        keeps the fastest; returns the chosen tuning with its timings."""
+        self._rc(self.lib.ksvd_autotune(self.h, _p(D), _p(X), _p(eps2), int(bool(force))))
+        t = self.get_tuning()
+        t["log"] = self.lib.ksvd_tuning_log(self.h).decode()
+        return t
+
     # -- allocation helpers -------------------------------------------------------
     def _dev(self):
         return torch.device("cuda", torch.cuda.current_device())
@@ -194,6 +218,8 @@ class Engine:
         return self.omp_stats()[1]
 
     def batch_omp_eps(self, DtX, G, xnorm2, eps2, out=None):
+        """Error-constrained Batch-OMP from DtX; xnorm2 (N,) float64 = ||x_s||^2."""
+        _check_shape(xnorm2, (self.N,), torch.float64, "xnorm2")
         idx, val, cnt = self.empty_compact() if out is None else out
         self._rc(self.lib.ksvd_batch_omp_eps(self.h, _p(DtX), _p(G), _p(xnorm2), _p(eps2), _p(idx), _p(val),
                                              _p(cnt)))
@@ -236,6 +262,17 @@ class Engine:
         rmse = (C.c_double * max(1, iternum))()
         self._rc(self.lib.ksvd_execute_iterations(self.h, _p(X), _p(D_in), _p(D_out), _p(idx), _p(val), _p(cnt),
                                                   int(iternum), rmse))
+        return D_out, idx, val, cnt, list(rmse)[:iternum]
+
+    def ExecuteIterations_eps(self, X, D_in, eps2, iternum, D_out=None, out=None):
+        """KSVD::ExecuteIterations with error-constrained Batch-OMP (stop at
+        ||r_s||^2 <= eps2[s] or T atoms).  Returns (D, idx, val, cnt, rmse)."""
+        _check_shape(eps2, (self.N,), torch.float32, "eps2")
+        D_out = torch.empty_like(D_in) if D_out is None else D_out
+        idx, val, cnt = self.empty_compact() if out is None else out
+        rmse = (C.c_double * max(1, iternum))()
+        self._rc(self.lib.ksvd_execute_iterations_eps(self.h, _p(X), _p(D_in), _p(D_out), _p(eps2), _p(idx), _p(val),
+                                                      _p(cnt), int(iternum), rmse))
         return D_out, idx, val, cnt, list(rmse)[:iternum]
 
     # -- differentiation ------------------------------------------------------------------

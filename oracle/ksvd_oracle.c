@@ -545,9 +545,21 @@ void FN(or_clear_dict)(int n, int K, int N, int T, const REAL* X, REAL* D, const
 /* D_out receives the working dictionary; idx/val/cnt the last Gamma;       */
 /* rmse[it] the per-iteration RMSE printed by the reference.                */
 /* ------------------------------------------------------------------------- */
+/* ExecuteIterations (S/KSVD.cu:55-131).  eps2 != NULL: the error-constrained */
+/* coding the comment at S/KSVD.cu:79 names (sparsecode with a threshold):   */
+/* or_batch_omp in error mode with ||x_s||^2 summed in double, T as T_max.   */
 int FN(or_ksvd)(int n, int K, int N, int T, const REAL* X, const REAL* Din, REAL* Dout, int iters,
                 uint64_t seed, double mu_thresh, int use_thresh, int max_signals, int32_t* idx,
-                REAL* val, int32_t* cnt, double* rmse) {
+                REAL* val, int32_t* cnt, double* rmse, const REAL* eps2) {
+    REAL* xn2 = NULL;
+    if (eps2) {
+        xn2 = (REAL*)malloc(sizeof(REAL) * (size_t)N);
+        for (long s = 0; s < N; ++s) {
+            double acc = 0;
+            for (int t = 0; t < n; ++t) acc += (double)X[(size_t)s * n + t] * (double)X[(size_t)s * n + t];
+            xn2[s] = (REAL)acc;
+        }
+    }
     FN(or_normcols)(n, K, Din, Dout);
     REAL* G = (REAL*)malloc(sizeof(REAL) * (size_t)K * K);
     REAL* DtX = (REAL*)malloc(sizeof(REAL) * (size_t)K * N);
@@ -562,7 +574,7 @@ int FN(or_ksvd)(int n, int K, int N, int T, const REAL* X, const REAL* Din, REAL
     for (int it = 0; it < iters; ++it) {
         AT_TIMES_B(n, K, K, Dout, Dout, G);
         AT_TIMES_B(n, K, N, Dout, X, DtX);
-        FN(or_batch_omp)(K, N, T, DtX, G, NULL, NULL, idx, val, cnt, NULL, NULL, NULL, NULL);
+        FN(or_batch_omp)(K, N, T, DtX, G, NULL, NULL, idx, val, cnt, xn2, eps2, NULL, NULL);
         FN(or_du_init)(N, K, unused, &usc, bitmap, replaced);
         FN(or_perm_shuffle)(perm, K, &state);
         FN(or_dict_update)(n, K, N, T, X, Dout, idx, val, cnt, perm, unused, &usc, bitmap, replaced,
@@ -572,7 +584,7 @@ int FN(or_ksvd)(int n, int K, int N, int T, const REAL* X, const REAL* Din, REAL
         FN(or_clear_dict)(n, K, N, T, X, Dout, idx, val, cnt, replaced, err, mu_thresh, use_thresh,
                           NULL);
     }
-    free(G); free(DtX); free(err); free(unused); free(bitmap); free(replaced); free(perm);
+    free(G); free(DtX); free(err); free(unused); free(bitmap); free(replaced); free(perm); free(xn2);
     return 0;
 }
 

@@ -69,7 +69,7 @@ class Oracle:
         self._du = fn("or_dict_update", None, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I)
         self._err = fn("or_error", _D, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P)
         self._clear = fn("or_clear_dict", None, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _D, _I, _P)
-        self._ksvd = fn("or_ksvd", _I, _I, _I, _I, _I, _P, _P, _P, _I, C.c_uint64, _D, _I, _I, _P, _P, _P, _P)
+        self._ksvd = fn("or_ksvd", _I, _I, _I, _I, _I, _P, _P, _P, _I, C.c_uint64, _D, _I, _I, _P, _P, _P, _P, _P)
         self._lsfix = fn("or_ls_fixed", None, _I, _I, _I, _I, _P, _P, _P, _P, _P)
         self._lsadj = fn("or_ls_adjoint", None, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P)
 
@@ -213,8 +213,10 @@ class Oracle:
                     _ptr(self._i(replaced)), _ptr(err), float(mu), int(use_thresh), _ptr(flags))
         return D, err, flags
 
-    def ksvd(self, X, Din, T, iters, seed=0, mu=0.8, use_thresh=4, max_signals=5000):
+    def ksvd(self, X, Din, T, iters, seed=0, mu=0.8, use_thresh=4, max_signals=5000, eps2=None):
+        """K-SVD iterations; eps2 (N,) switches Batch-OMP to the error-constrained stop."""
         X = self._f(X)
+        e2 = self._f(eps2) if eps2 is not None else None
         Din = self._f(Din)
         N, n = X.shape
         K = Din.shape[0]
@@ -224,7 +226,7 @@ class Oracle:
         cnt = np.empty(N, np.int32)
         rmse = np.empty(iters, np.float64)
         self._ksvd(n, K, N, T, _ptr(X), _ptr(Din), _ptr(Dout), iters, int(seed), float(mu), int(use_thresh),
-                   int(max_signals), _ptr(idx), _ptr(val), _ptr(cnt), _ptr(rmse))
+                   int(max_signals), _ptr(idx), _ptr(val), _ptr(cnt), _ptr(rmse), _ptr(e2))
         return {"D": Dout, "idx": idx, "val": val, "cnt": cnt, "rmse": rmse}
 
     def ls_fixed(self, D, X, idx, cnt):

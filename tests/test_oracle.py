@@ -417,3 +417,19 @@ def test_reference_fixture_structure(o64):
     assert np.all(res["cnt"] == 6)
     assert np.all(np.bincount(res["idx"].ravel(), minlength=9) > 0)
     assert np.all(np.isfinite(res["val"]))
+
+
+def test_ksvd_error_mode_with_zero_threshold_is_sparsity_mode(o64):
+    """eps^2 = 0 never stops a signal whose residual is not exactly zero, so
+    the error-constrained loop reduces to the sparsity-mode loop."""
+    X, wl = c0(300, seed=8)
+    D0 = wl.D_init.numpy().astype(np.float64) + 0.01  # no signal is an atom
+    a = o64.ksvd(X, D0, 6, 2, seed=3)
+    b = o64.ksvd(X, D0, 6, 2, seed=3, eps2=np.zeros(len(X)))
+    # (the RMSE total is an OpenMP reduction: last-bit differences between runs)
+    np.testing.assert_allclose(a["rmse"], b["rmse"], rtol=1e-12)
+    np.testing.assert_array_equal(a["D"], b["D"])
+    np.testing.assert_array_equal(a["cnt"], b["cnt"])
+    # and a threshold above every signal's energy codes nothing
+    c = o64.ksvd(X, D0, 6, 1, seed=3, eps2=np.full(len(X), 1e30))
+    assert np.all(c["cnt"] == 0)
