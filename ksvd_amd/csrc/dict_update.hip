@@ -5,18 +5,22 @@
 // launches ~12 kernels per atom including a full n x N x K cuBLAS GEMM for the
 // unused-atom test, and builds its per-atom lists with atomicAdd slots
 // (nondeterministic order).  This engine works on the compact Gamma
-// (idx/val/cnt), builds the per-atom lists once per sweep in ascending signal
-// order with a two-level counting sort (no order-dependent atomics), and runs
-// five small launches per atom whose work is proportional to the atom's list
-// length N_p (not N or K x N):
-//   1. special-case error of the first min(N, max_signals) unused signals
-//      (only when N_p = 0; sparse reconstruction, no GEMM)
-//   2. special-case arg-max + replacement
-//   3. v = Gamma(:, I_p) gamma_p^T  -- per-block LDS accumulation
-//   4. a = Y_p - D v + d_p ||gamma_p||^2, ||a||, u = D^T a  -- one workgroup
-//   5. gamma_new = (a^T x - u^T Gamma(:,s) + u_p gamma)/||a||, d_p = a/||a||
-// All data-dependent branching is device-side (early exits on N_p), so the
-// host loop never synchronises and the sequence is graph-capturable.
+// (idx/val/cnt) and the restricted residuals R = X - D Gamma, formed once per
+// sweep (du_residual4_kernel).  The per-atom lists are built once per sweep in
+// ascending signal order by a two-level counting sort (no order-dependent
+// atomics).  For n <= 1024 the sweep is pipelined by one atom, two launches
+// per atom:
+//   du_ba_kernel(jj)       phase B of the atom at sweep position jj
+//                          (gamma_new = d_new^T (r_s + d_p gamma_s), residual
+//                          update), fused with phase A of the atom at jj + 1
+//                          (its block partials of a = E gamma^T), and the
+//                          column commit of the atom at jj - 1, one element per
+//                          workgroup; an unused atom takes the special case
+//                          (sparse errors from R, no GEMM) here;
+//   du_a_reduce_kernel     the fixed-order sum of those partials -> a, ||a||.
+// Above n = 1024 the unpipelined du_phase_a/b kernels do the same work.  All
+// data-dependent branching is device-side (early exits on N_p), so the host
+// loop never synchronises and the sequence is graph-capturable.
 #include <algorithm>
 
 #include "common.hpp"
@@ -470,9 +474,9 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
 // gamma) lane-parallel, then streams batches of residual rows (one float4 per
 // lane per 256 columns); 8 waves per block.  Phase A's block partials are
 // reduced by du_a_reduce_kernel in a fixed order (deterministic).  Phase B
-// does not write d_p: the next launch (du_ba_kernel's block 0) copies the new
-// column in, and du_tail_kernel does it for the last atom of the sweep;
-// nothing reads d_p in between.
+// does not write d_p: the next launch commits the new column (thread 0 of
+// workgroups 0 .. n-1, one element each), and du_tail_kernel does it for the
+// last atom of the sweep; nothing reads d_p in between.
 constexpr int DU_WAVES = 8;
 constexpr int FQ = 8;   // rows per batch in du_ba_kernel, divided by the float4 chunks per row
 constexpr int FQA = 8;  // residual rows in flight per wave in du_a_kernel (phase A of the first atom)

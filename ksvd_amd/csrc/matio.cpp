@@ -149,9 +149,11 @@ int parse_matrix(const Reader& r, size_t data, uint32_t nbytes, const char* name
     if (ndim != 2) return fail(KSVD_ERR_INVALID, "MAT: variable '" + vname + "' is not 2-D");
     // real part
     if (!read_tag(m, off, t, nb, d, tot)) return fail(KSVD_ERR_INVALID, "MAT: bad real-part subelement");
-    const size_t count = (size_t)dims[0] * (size_t)dims[1];
-    const size_t ts = type_size(t);
-    if (ts == 0 || (size_t)nb != count * ts) return fail(KSVD_ERR_INVALID, "MAT: real part has the wrong size");
+    if (dims[0] < 0 || dims[1] < 0) return fail(KSVD_ERR_INVALID, "MAT: variable '" + vname + "' has a negative dimension");
+    // both dimensions are below 2^31, so the count and its byte size fit 64 bits
+    const uint64_t count = (uint64_t)dims[0] * (uint64_t)dims[1];
+    const uint64_t ts = type_size(t);
+    if (ts == 0 || (uint64_t)nb != count * ts) return fail(KSVD_ERR_INVALID, "MAT: real part has the wrong size");
     if (!to_double(m.p + d, t, count, m.swap, values)) return fail(KSVD_ERR_INVALID, "MAT: unsupported data type");
     rows = dims[0];
     cols = dims[1];

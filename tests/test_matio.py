@@ -86,6 +86,22 @@ def test_reader_errors(tmp_path):
         _lib.mat_read(str(tmp_path / "absent.mat"), "A")
 
 
+def test_reader_rejects_negative_dimensions(tmp_path):
+    """A crafted file whose dimensions are -1 x -8 over 8 doubles: the product
+    of the two negative sizes equals the element count, so only an explicit
+    sign check stops it."""
+    p = str(tmp_path / "neg.mat")
+    sio.savemat(p, {"A": np.arange(8.0).reshape(1, 8)}, do_compression=False)
+    raw = bytearray(open(p, "rb").read())
+    # 128-byte header, miMATRIX tag (8), array-flags subelement (16), dims tag (8)
+    off = 128 + 8 + 16 + 8
+    assert np.frombuffer(bytes(raw[off:off + 8]), "<i4").tolist() == [1, 8]
+    raw[off:off + 8] = np.array([-1, -8], "<i4").tobytes()
+    open(p, "wb").write(bytes(raw))
+    with pytest.raises(_lib.KsvdError, match="negative dimension"):
+        _lib.mat_read(p, "A")
+
+
 def test_driver_arguments():
     assert os.path.exists(MAIN), "ksvd_main not built"
     r = subprocess.run([MAIN, "--help"], capture_output=True, text=True, timeout=60)
