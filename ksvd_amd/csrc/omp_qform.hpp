@@ -24,8 +24,9 @@
 // the workgroup's register files (and, for the largest basis, LDS) instead
 // of being re-streamed from L2.  All steps are unrolled, so every index into
 // Q is a compile-time constant.
-// Per step, one barrier: each wave's arg-max (DPP max + ballot) goes to LDS
-// as one 12-byte record (alpha^2 bits, atom, alpha) with its winning lane's
+// Per step, one barrier: each wave's arg-max of |alpha| (max3 on magnitudes,
+// DPP max, ballot) goes to LDS as one 12-byte record (|alpha| bits, atom,
+// alpha) with its winning lane's
 // row of Q (double-buffered by step parity) -> barrier -> lanes w < W of
 // every wave read the W records and reduce them by DPP (same data, same
 // order: identical decisions, so every wave leaves the loop at the same
@@ -94,6 +95,10 @@ __device__ __forceinline__ void load_slice(const float* __restrict__ p, int kb, 
 // ~26 others): three waves, or four for the 48-slot basis (TT*KPL = 96, a
 // few spilled registers, measured faster than three waves without spills).
 constexpr bool qform_split(int kpl, int tt, bool eps) { return kpl == 4 && tt == 32 && !eps; }
+// |alpha| >= this float (bits) <=> (double)fl32(alpha^2) >= 1e-14, the
+// reference's stop test (S/OMP.cu:292), fl32 squaring being monotone:
+// 0x33D6BF96 is the smallest float whose rounded square reaches 1e-14.
+constexpr int kAbsAlphaMin = 0x33D6BF96;
 constexpr int qform_reg_cols(int kpl, int tt, bool eps) { return qform_split(kpl, tt, eps) ? 22 : tt - 1; }
 constexpr int qform_waves_per_eu(int kpl, int tt, bool eps) {
     return tt * kpl < 96 ? 1 : qform_split(kpl, tt, eps) ? 4 : 512 / (tt * kpl + 3 * kpl + 26);
@@ -136,7 +141,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
     constexpr int NG = TT / UG;  // groups of UG support slots
     __shared__ float Ls[TT * (TT + 1) / 2];
     __shared__ float ys[TT];       // y = L^-1 alpha0_I
-    // per step parity: each wave's candidate (alpha^2, atom, signed alpha) and
+    // per step parity: each wave's candidate (|alpha|, atom, signed alpha) and
     // the candidate's row of Q (zero beyond the step), which is w if it wins
     __shared__ int3 cand[2][W];
     __shared__ __align__(16) float wrow[2][W][TT];  // rows < QR used
@@ -210,15 +215,14 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
                 if (xn2 - ysum <= e2) break;
             }
             const int b = i & 1;  // step parity
-            // this wave's arg-max of alpha^2: largest value, lowest atom on ties
+            // this wave's arg-max of |alpha|: largest value, lowest atom on ties
             // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges).
-            // alpha^2 >= 0, so its bit pattern orders like a signed int.
-            int c2[KPL];
+            // |alpha| >= 0, so its bit pattern orders like a signed int; the
+            // magnitudes come from the max instructions' abs modifiers
+            float mf = fabsf(al[0]);
 #pragma unroll
-            for (int m = 0; m < KPL; ++m) c2[m] = __float_as_int(al[m] * al[m]);
-            int bv = -1;
-#pragma unroll
-            for (int m = 0; m < KPL; ++m) bv = max(bv, c2[m]);
+            for (int m = 1; m < KPL; ++m) mf = fmaxf(mf, fabsf(al[m]));
+            const int bv = __float_as_int(mf);
             const int M = wave_max_i32_bcast(bv);
             const int wl = __builtin_ctzll(__builtin_amdgcn_ballot_w64(bv == M));
             if (lane == wl) {
@@ -226,7 +230,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
                 int bm = KPL - 1;
 #pragma unroll
                 for (int m = KPL - 2; m >= 0; --m)
-                    if (c2[m] == bv) bm = m;
+                    if (fabsf(al[m]) == mf) bm = m;
                 float av = al[0];
 #pragma unroll
                 for (int m = 1; m < KPL; ++m)
@@ -255,25 +259,23 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             // the block's arg-max over the W records, lanes w < W holding
             // record w: max by DPP, then the lowest wave holding it (the lowest
             // atom among equal values)
-            int cv = INT_MIN, ck = 0, ca = 0;
-            if (lane < W) {
-                const int3 r = cand[b][lane & (W - 1)];
-                cv = r.x; ck = r.y; ca = r.z;
-            }
+            // every lane reads record lane mod W (broadcast reads): each group
+            // of W lanes holds all W records, so the lowest lane holding the
+            // maximum is below W
+            const int3 rec = cand[b][lane & (W - 1)];
+            const int cv = rec.x, ck = rec.y, ca = rec.z;
             int mx = cv;
             if constexpr (W >= 2) mx = max(mx, dpp_i<0xB1>(mx));
             if constexpr (W >= 4) mx = max(mx, dpp_i<0x4E>(mx));
             if constexpr (W >= 8) mx = max(mx, dpp_i<0x141>(mx));
             const int best = __builtin_amdgcn_readfirstlane(mx);
-            // (lanes >= W hold INT_MIN, below every record)
-            const int vo = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(cv == best) | (1ull << 63)) & 63;
+            const int vo = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(cv == best));
             const int pos = __builtin_amdgcn_readlane(ck, vo);
             const float apos = __int_as_float(__builtin_amdgcn_readlane(ca, vo));
             // stop: inner product too small, or selected twice (S/OMP.cu:292,304);
             // pos >= K: only padding atoms left (every real alpha NaN or 0)
-            // ((double)alpha^2 >= 1e-14 holds exactly for the bit patterns from
-            // 0x283424DD up to +inf)
-            if ((unsigned)pos >= (unsigned)K || (unsigned)(best - 0x283424DD) > (unsigned)(0x7F800000 - 0x283424DD))
+            if ((unsigned)pos >= (unsigned)K ||
+                (unsigned)(best - kAbsAlphaMin) > (unsigned)(0x7F800000 - kAbsAlphaMin))
                 break;
             if ((__builtin_amdgcn_ballot_w64(sp.ind == pos) & ((1ull << i) - 1ull)) != 0ull) break;
             float g[KPL];
