@@ -58,7 +58,8 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
     tmp = MAIN + ".tmp"
-    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "ksvd_main.cpp"), "-o", tmp, f"-L{HERE}",
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "ksvd_main.cpp"), "-o", tmp,
+           f"-L{HERE}",
            "-lksvd_amd", "-Wl,-rpath,$ORIGIN"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

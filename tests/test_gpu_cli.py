@@ -67,6 +67,41 @@ def test_driver_end_to_end(o64, tmp_path):
     eng.close()
 
 
+def test_driver_phases_match_execute_iterations(tmp_path):
+    """The C++ facade (include/ksvd_amd.hpp) driven phase by phase -- normcols,
+    then per iteration the Gram, BatchOMP, initialize, updateDictionary,
+    errorComputation, clearDict (ksvd_main --phases) -- against its
+    KSVD::ExecuteIterations (the default): the same atom orders and kernels;
+    only the error is formed differently (from X and D instead of the
+    maintained residuals), so RMSE and dictionaries agree to rounding."""
+    say("ksvd_main --phases vs ExecuteIterations, C0 N=2000, 3 iterations")
+    wl = synth.make("C0", N=2000)
+    X = wl.X.numpy()
+    rng = np.random.default_rng(5)
+    D0 = wl.D_init.numpy()
+    D0 = (D0 + 0.05 / 8 * np.linalg.norm(D0, axis=1, keepdims=True) * rng.standard_normal(D0.shape)).astype(np.float32)
+    (tmp_path / "input").mkdir()
+    (tmp_path / "output").mkdir()
+    _lib.mat_write(str(tmp_path / "input" / "X.mat"), {"X": X.T.astype(np.float64)})
+    _lib.mat_write(str(tmp_path / "input" / "D.mat"), {"D": D0.T.astype(np.float64)})
+    outs = {}
+    for mode in ("loop", "phases"):
+        args = [MAIN, "--iters", "3", "--T", "8", "--seed", "4", "--out", f"output/{mode}.mat",
+                "--dict-out", f"output/{mode}_D.mat"] + (["--phases"] if mode == "phases" else [])
+        r = subprocess.run(args, cwd=str(tmp_path), capture_output=True, text=True, timeout=180)
+        say(mode, r.stdout.replace("\n", " | ")[-300:])
+        assert r.returncode == 0, r.stderr
+        rm = [float(m) for m in re.findall(r"Iteration \d\d / 3 complete, RMSE = ([0-9.]+)", r.stdout)]
+        assert len(rm) == 3
+        outs[mode] = (rm, sio.loadmat(str(tmp_path / "output" / f"{mode}_D.mat"))["FinalD"],
+                      sio.loadmat(str(tmp_path / "output" / f"{mode}.mat"))["CUDA_Gamma"])
+    (ra, Da, Ga), (rb, Db, Gb) = outs["loop"], outs["phases"]
+    np.testing.assert_allclose(rb, ra, rtol=0, atol=2e-5)  # printed with 5 decimals
+    np.testing.assert_allclose(Db, Da, rtol=0, atol=1e-5)
+    assert np.array_equal(Ga != 0, Gb != 0)
+    np.testing.assert_allclose(Gb, Ga, rtol=0, atol=1e-4)
+
+
 def test_compare_mode():
     """bench.py --compare (the fmriKSVD.m check, Matlab/fmriKSVD.m:26-70): CPU FP32 port
     and GPU engine on the same problem end at the same RMSE, iteration by iteration."""
