@@ -363,8 +363,8 @@ int time_ms(ksvd_handle* h, int reps, double* ms, F&& fn) {
         if (rc == KSVD_OK) rc = hip_check(h, hipEventElapsedTime(&t, e0, e1), "elapsed");
         if (rc == KSVD_OK && t < best) best = t;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     *ms = best;
     return rc;
 }
@@ -526,8 +526,8 @@ int autotune(ksvd_handle* h, const float* D, const float* X, const float* eps2, 
             HIPTRY(h, hipEventSynchronize(e1), "sync");
             float t = 0.f;
             HIPTRY(h, hipEventElapsedTime(&t, e0, e1), "elapsed");
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
             one = t;
             if (rep > 0 && one < ms) ms = one;
         }
@@ -595,14 +595,14 @@ int ksvd_create(ksvd_handle** out, const ksvd_config* cfg) {
 
 int ksvd_destroy(ksvd_handle* h) {
     if (!h) return KSVD_ERR_NULL;
-    if (!h->allocs.empty()) hipStreamSynchronize(h->stream);
+    if (!h->allocs.empty()) (void)hipStreamSynchronize(h->stream);
     if (h->gstream) {
-        hipStreamSynchronize(h->gstream);
-        hipStreamDestroy(h->gstream);
+        (void)hipStreamSynchronize(h->gstream);
+        (void)hipStreamDestroy(h->gstream);
     }
-    if (h->ev_in) hipEventDestroy(h->ev_in);
-    if (h->ev_out) hipEventDestroy(h->ev_out);
-    for (void* p : h->allocs) hipFree(p);
+    if (h->ev_in) (void)hipEventDestroy(h->ev_in);
+    if (h->ev_out) (void)hipEventDestroy(h->ev_out);
+    for (void* p : h->allocs) (void)hipFree(p);
     delete h;
     return KSVD_OK;
 }
@@ -930,10 +930,10 @@ static int execute_iterations(ksvd_handle* h, const float* X, const float* D_in,
         if (rc == KSVD_OK) rc = stage_out(it);
     }
     if (exec) {
-        hipStreamSynchronize(h->gstream);
-        hipGraphExecDestroy(exec);
+        (void)hipStreamSynchronize(h->gstream);
+        (void)hipGraphExecDestroy(exec);
     }
-    if (graph) hipGraphDestroy(graph);
+    if (graph) (void)hipGraphDestroy(graph);
     if (rc != KSVD_OK) return rc;
     if (rmse_host && iterations > 0) {
         HIPTRY(h, hipMemcpyAsync(rmse_host, h->rmse_all, sizeof(double) * iterations, hipMemcpyDeviceToHost, h->stream),

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(64) void du_fill_kernel(const int32_t* __restrict__
         }
 #pragma unroll
         for (int u = 0; u < FB; ++u) {
-            if (b + u >= ns) break;
+            if (b + u >= ns) continue;  // (uniform) past the chunk's last signal
             const long s = s0 + b + u;
             const int cc = readlane(cl, u);
             const int kk = (lane < cc) ? kq[u] : 0;

@@ -148,10 +148,11 @@ hipError_t launch_kpl(const OmpArgs& a, hipStream_t st) {
     const size_t lds = (size_t)WPB * a.T * (a.T + 1) * sizeof(float) + (size_t)WPB * KPL * 64;
     if (a.eps2) {
         int dev = 0, cus = 0;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         int per_cu = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, omp_wave_kernel<KPL, true>, 256, lds);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, omp_wave_kernel<KPL, true>, 256, lds);
+        if (e != hipSuccess) return e;
         if (per_cu < 1) per_cu = 1;
         long want = (a.N + WPB - 1) / WPB;
         long grid = (long)cus * per_cu;
