@@ -176,6 +176,61 @@ def test_c2_supports_true_dictionary(o64, form):
 
 
 @pytest.mark.parametrize("variant", ["wave", "qform"])
+@pytest.mark.parametrize("n,K,N,T", [(32, 40, 7, 8), (64, 256, 1, 8), (64, 256, 50, 1), (48, 48, 20, 48)])
+def test_batch_omp_degenerate_inputs(o64, variant, n, K, N, T):
+    """Degenerate inputs: all-zero signals (empty support: alpha^2 < 1e-14 at
+    step 0, S/OMP.cu:292), a signal equal to an atom (that atom, coefficient 1,
+    then an exact fit), a batch of one signal, T = 1, fewer atoms than a wave
+    has lanes (padding lanes), and T = K with n = K (every atom usable, the
+    residual exhausted)."""
+    rng = np.random.default_rng(7919 * n + 31 * K + 17 * N + T)
+    D = rng.standard_normal((K, n))
+    D = (D / np.linalg.norm(D, axis=1, keepdims=True)).astype(np.float32)
+    X = rng.standard_normal((N, n)).astype(np.float32)
+    zero_rows = ([0] if N > 1 else []) + ([N - 2] if N > 3 else [])
+    atom_row, atom = (N - 1, min(5, K - 1)) if N > 1 else (None, None)
+    for r in zero_rows:
+        X[r] = 0.0
+    if atom_row is not None:
+        X[atom_row] = D[atom]
+    say(f"{variant} n={n} K={K} N={N} T={T}: zero rows {zero_rows}, atom row {atom_row}")
+    eng = Engine(n, K, N, T, omp_variant=variant)
+    Dg = cu(D)
+    G = eng.gram(Dg)
+    idx, val, cnt = [host(t) for t in eng.batch_omp_compact(eng.correlate(Dg, cu(X)), G)]
+    eng.close()
+    X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    A064 = o64.correlate(D64, X64)
+    ref = o64.batch_omp(A064, o64.gram(D64), T, gaps=True)
+    for r in zero_rows:
+        assert cnt[r] == 0 and ref["cnt"][r] == 0
+        assert np.all(idx[r] == -1) and np.all(val[r] == 0.0)
+    if atom_row is not None:
+        assert idx[atom_row, 0] == atom and abs(val[atom_row, 0] - 1.0) <= 1e-5
+        assert np.abs(val[atom_row, 1:cnt[atom_row]]).max(initial=0.0) <= 1e-3
+        assert ref["idx"][atom_row, 0] == atom
+    assert np.all(cnt <= T) and np.all(cnt >= 0)
+    for s in range(N):
+        assert np.all(idx[s, cnt[s]:] == -1)
+        sup = idx[s, :cnt[s]]
+        assert len(np.unique(sup)) == len(sup) and np.all((sup >= 0) & (sup < K))
+    # coefficients: FP32 least squares on the support, so the error bound
+    # scales with the condition number of G_II (large when T = K = n: the
+    # support spans the whole space): 1e-4, or 20 kappa 2^-24 when larger
+    _check_omp((idx, val, cnt), ref, A064, coef_tol=np.inf, max_tie_frac=0.1)
+    worst = 0.0
+    for s in range(N):
+        c = int(cnt[s])
+        if c == 0 or c != ref["cnt"][s] or not np.array_equal(idx[s, :c], ref["idx"][s, :c]):
+            continue
+        kappa = np.linalg.cond(D64[idx[s, :c]] @ D64[idx[s, :c]].T)
+        rel = np.linalg.norm(val[s, :c] - ref["val"][s, :c]) / np.linalg.norm(ref["val"][s, :c])
+        worst = max(worst, rel / max(1e-4, 20 * kappa * 2.0 ** -24))
+    say(f"coefficients: worst error / its bound {worst:.3f}")
+    assert worst <= 1.0
+
+
+@pytest.mark.parametrize("variant", ["wave", "qform"])
 def test_batch_omp_dense_reference_layout(o64, variant):
     """SELECTED_ATOMS slot map and c exactly as parOMP writes them, including the
     pivot-break slot that points at a stale c entry."""
