@@ -307,6 +307,9 @@ __global__ __launch_bounds__(256) void du_phase_a_kernel(int jj, const uint32_t*
     if (Np == 0) {
         const int counter = *usc;
         const int lim = min(M, N - counter);
+        // every signal already replaced an atom this sweep: no candidate, the
+        // atom stays as it is (the reference would read before its unused list)
+        if (lim <= 0) return;
         float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
         int bi = 0;
         for (int b = blockIdx.x * 4 + w; b < lim; b += G * 4) {
@@ -540,6 +543,10 @@ __device__ void du_special_case(int p, const float* __restrict__ X, float* __res
     const int G = gridDim.x;
     const int counter = *usc;
     const int lim = min(M, N - counter);
+    // every signal already replaced an atom this sweep: no candidate, the atom
+    // stays as it is (the reference would read before its unused list); the
+    // same value in every block, so no block takes the ticket below
+    if (lim <= 0) return;
     float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
     int bi = 0;
     for (int b = blockIdx.x * DU_WAVES + w; b < lim; b += G * DU_WAVES) {

@@ -409,6 +409,11 @@ void FN(or_dict_update)(int n, int K, int N, int T, const REAL* X, REAL* D, cons
             /* S/ReductionBase.cuh:176-180); error vs the current D*Gamma.       */
             int live = N - *us_counter;
             int M = N < max_signals ? N : max_signals;
+            /* every signal already replaced an atom in this sweep: no candidate */
+            /* left, the atom stays as it is (the reference's reduction over an  */
+            /* empty range returns index 0 and it then reads unused[-1]: no      */
+            /* defined result to follow)                                         */
+            if (live <= 0) continue;
             int best = 0;
             REAL bestv = 0;
             for (int b = 0; b < M && b < live; ++b) {

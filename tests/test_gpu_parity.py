@@ -336,6 +336,41 @@ def test_dictionary_update_parity(o64, cfg, N, max_signals):
     eng.close()
 
 
+@pytest.mark.parametrize("n", [24, 1100])
+def test_dictionary_update_unused_list_exhausted(o64, n):
+    """More unused atoms in one sweep than signals (K = 64 atoms, N = 10
+    signals, T = 2): the special case consumes every signal of the unused
+    list, after which no candidate is left and the remaining unused atoms stay
+    as they are (the reference's reduction over an empty range would make it
+    read before its list).  Both the pipelined sweep and the unpipelined one
+    (n > 1024)."""
+    K, N, T = 64, 10, 2
+    rng = np.random.default_rng(n + 64)
+    D = H.unit_rows(rng.standard_normal((K, n))).astype(np.float32)
+    X = rng.standard_normal((N, n)).astype(np.float32)
+    X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    r = o64.batch_omp(o64.correlate(D64, X64), o64.gram(D64), T)
+    idx, val, cnt = r["idx"].copy(), r["val"].astype(np.float32), r["cnt"].copy()
+    unused = np.setdiff1d(np.arange(K), idx[idx >= 0])
+    assert len(unused) > N
+    perm = np.random.default_rng(2).permutation(K).astype(np.uint32)
+    eng = Engine(n, K, N, T)
+    Dg, valg = cu(D), cu(val)
+    eng.du_init()
+    eng.updateDictionary(cu(X), Dg, cu(idx, torch.int32), valg, cu(cnt, torch.int32), perm)
+    D1, v1 = host(Dg), host(valg)
+    eng.close()
+    Dr, vr, st = o64.dict_update(X64, D64, idx, val.astype(np.float64), cnt, perm)
+    assert st["usc"][0] == N  # every signal replaced an atom
+    order = [int(p) for p in perm if p in set(unused.tolist())]
+    kept = order[N:]  # unused atoms met after the list ran out
+    np.testing.assert_array_equal(D1[kept], D[kept])
+    np.testing.assert_array_equal(Dr[kept], D64[kept])
+    say(f"n={n}: {len(unused)} unused atoms, {N} replaced, {len(kept)} kept; D max abs diff {np.abs(D1 - Dr).max():.2e}")
+    np.testing.assert_allclose(D1, Dr, atol=1e-4)
+    np.testing.assert_allclose(v1, vr, atol=1e-4 * max(1.0, np.abs(vr).max()))
+
+
 @pytest.mark.parametrize("n,K,N,T", [(130, 96, 1500, 6), (400, 128, 1200, 8), (700, 160, 1200, 8),
                                      (1100, 128, 800, 6)])
 def test_dictionary_update_parity_signal_lengths(o64, n, K, N, T):
