@@ -352,6 +352,7 @@ __global__ __launch_bounds__(256) void du_phase_a_kernel(int jj, const uint32_t*
             sq += x * x;
         }
         sq = block_sum(sq, wred);
+        if (sq == 0.f) return;  // a zero signal (every candidate error 0): the atom stays
         const float nrm = sqrtf(sq);
         for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
         __syncthreads();
@@ -589,6 +590,9 @@ __device__ void du_special_case(int p, const float* __restrict__ X, float* __res
         sq += x * x;
     }
     sq = block_sum(sq, wred);
+    // the pick is a zero signal (every candidate error 0): x / ||x|| is
+    // undefined, the atom stays as it is
+    if (sq == 0.f) return;
     const float nrm = sqrtf(sq);
     for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
     __syncthreads();
@@ -1135,13 +1139,16 @@ __global__ __launch_bounds__(1024) void clear_sweep_kernel(const float* __restri
         }
         __syncthreads();
         const long best = ri[0];
-        ++nrep;
         float sq = 0.f;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const float x = X[best * n + t];
             sq += x * x;
         }
         sq = block_sum(sq, wred);
+        // the pick is a zero signal (every remaining error 0): x / ||x|| is
+        // undefined, the atom stays as it is
+        if (sq == 0.f) continue;
+        ++nrep;
         const float nrm = sqrtf(sq);  // EUnorm S/ReductionKernels.cu:413-430
         for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)j * n + t] = X[best * n + t] / nrm;
         __threadfence_block();

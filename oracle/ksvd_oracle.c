@@ -427,6 +427,9 @@ void FN(or_dict_update)(int n, int K, int N, int T, const REAL* X, REAL* D, cons
             REAL nrm = 0;
             for (int t = 0; t < n; ++t) nrm += SQR(X[(size_t)sig * n + t]);
             nrm = sqrt(nrm);
+            /* a zero signal (every candidate error 0): x / ||x|| is undefined,  */
+            /* the atom stays as it is                                           */
+            if (nrm == 0) continue;
             for (int t = 0; t < n; ++t) D[(size_t)p * n + t] = X[(size_t)sig * n + t] / nrm;
             int counter = *us_counter;
             unused[best] = unused[N - counter - 1];
@@ -540,6 +543,7 @@ void FN(or_clear_dict)(int n, int K, int N, int T, const REAL* X, REAL* D, const
         REAL nrm = 0;
         for (int t = 0; t < n; ++t) nrm += SQR(X[(size_t)best * n + t]);
         nrm = sqrt(nrm);
+        if (nrm == 0) continue; /* a zero signal: the atom stays as it is */
         for (int t = 0; t < n; ++t) D[(size_t)j * n + t] = X[(size_t)best * n + t] / nrm;
     }
     free(Gj); free(usecount);

@@ -371,6 +371,43 @@ def test_dictionary_update_unused_list_exhausted(o64, n):
     np.testing.assert_allclose(v1, vr, atol=1e-4 * max(1.0, np.abs(vr).max()))
 
 
+@pytest.mark.parametrize("n", [24, 1100])
+def test_zero_signal_replacement_candidates(o64, n):
+    """Replacement candidates that are zero signals: the unused-atom special
+    case looks at the first max_signals = 5 entries of the unused-signal list,
+    here five all-zero signals (every error 0, so the pick is the first), and
+    clearDict runs out of represented signals; x / ||x|| is undefined for a
+    zero signal, so the atom stays as it is instead of turning into NaN.
+    Dictionary update on both sweep forms, then the whole loop."""
+    K, N, T = 64, 20, 2
+    rng = np.random.default_rng(n + 5)
+    D = H.unit_rows(rng.standard_normal((K, n))).astype(np.float32)
+    X = rng.standard_normal((N, n)).astype(np.float32)
+    X[:5] = 0.0
+    X64, D64 = X.astype(np.float64), D.astype(np.float64)
+    r = o64.batch_omp(o64.correlate(D64, X64), o64.gram(D64), T)
+    idx, val, cnt = r["idx"].copy(), r["val"].astype(np.float32), r["cnt"].copy()
+    perm = np.random.default_rng(3).permutation(K).astype(np.uint32)
+    eng = Engine(n, K, N, T, max_signals=5)
+    Dg, valg = cu(D), cu(val)
+    eng.du_init()
+    eng.updateDictionary(cu(X), Dg, cu(idx, torch.int32), valg, cu(cnt, torch.int32), perm)
+    D1, v1 = host(Dg), host(valg)
+    Dr, vr, st = o64.dict_update(X64, D64, idx, val.astype(np.float64), cnt, perm, max_signals=5)
+    assert np.isfinite(D1).all() and np.isfinite(Dr).all()
+    assert st["usc"][0] == 0  # no replacement: every candidate was a zero signal
+    np.testing.assert_allclose(D1, Dr, atol=1e-4)
+    np.testing.assert_allclose(v1, vr, atol=1e-4 * max(1.0, np.abs(vr).max()))
+    # the whole loop (clearDict included): finite, and the oracle's RMSE
+    Dl, _, _, _, rmse = eng.ExecuteIterations(cu(X), cu(D), 2)
+    ref = o64.ksvd(X64, D64, T, 2, seed=0, max_signals=5)
+    eng.close()
+    say(f"n={n}: rmse {rmse} oracle {list(ref['rmse'])}")
+    assert np.isfinite(host(Dl)).all() and np.all(np.isfinite(rmse))
+    # (iteration 1 fits every signal exactly: RMSE ~1e-16 in FP64, ~1e-8 in FP32)
+    np.testing.assert_allclose(rmse, ref["rmse"], rtol=1e-4, atol=1e-6)
+
+
 @pytest.mark.parametrize("n,K,N,T", [(130, 96, 1500, 6), (400, 128, 1200, 8), (700, 160, 1200, 8),
                                      (1100, 128, 800, 6)])
 def test_dictionary_update_parity_signal_lengths(o64, n, K, N, T):
