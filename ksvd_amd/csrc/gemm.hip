@@ -35,7 +35,16 @@ constexpr int BM = 128;    // tile extent along colsB (MFMA rows)
 constexpr int BN = 128;    // tile extent along colsA (MFMA cols)
 constexpr int BK = 32;     // reduction chunk
 constexpr int PITCH = 36;  // LDS row pitch in floats (pads 128 B rows: no bank conflicts)
-constexpr int PH = 40;     // LDS row pitch in bf16 elements for the BF16 forms (80 B rows)
+constexpr int PH = 32;     // LDS row pitch in bf16 elements for the BF16 forms (64 B rows, swizzled)
+
+// BF16 tiles: a 64-byte row is four 16-byte units, and unit u of row r is
+// stored at unit u ^ bf16_swz(r).  A bf16x8 MFMA operand read has lanes
+// 0..31 on 32 consecutive rows at one unit; the swizzle puts any 8 (or 16)
+// consecutive rows at distinct 16-byte bank groups of a 128 (or 256) byte
+// bank row, so the reads are conflict-free without padding the pitch, and
+// the split-BF16x6 form's six tiles take 48 kB (three workgroups per CU)
+// instead of 60 kB.
+__device__ __forceinline__ int bf16_swz(int r) { return ((r >> 1) & 3) ^ ((r >> 3) & 1); }
 
 // One 128 x 32 tile (rows of the tile are columns of the matrix): each
 // thread fetches 4 float4s into registers (fetch_tile), later stored to LDS
@@ -95,7 +104,9 @@ __device__ __forceinline__ void put_tile_bf16(__bf16* __restrict__ sh, __bf16* _
                 l[e] = (__bf16)(r1 - (float)me);
             }
         }
-        const int off = (q >> 3) * PH + (q & 7) * 4;
+        // row q / 8, unit (q % 8) / 2, half q % 2 of the unit
+        const int row = q >> 3;
+        const int off = row * PH + ((((q & 7) >> 1) ^ bf16_swz(row)) << 3) + (q & 1) * 4;
         *reinterpret_cast<bf16x4*>(sh + off) = h;
         if constexpr (FORM >= 2) *reinterpret_cast<bf16x4*>(sl + off) = l;
         if constexpr (FORM == 3) *reinterpret_cast<bf16x4*>(sm + off) = m;
@@ -114,8 +125,8 @@ __device__ __forceinline__ void put_tile(float* __restrict__ s, const float4* v)
 // Compiled for WPE workgroups per CU (3 by default: the split form needs 192
 // registers per lane at the compiler's default and would run two -- 3.03 ms
 // against 2.70 ms for the n=256, K=1024, N=10^6 correlation, with a few
-// spilled registers; the FP32 and BF16 forms fit three either way; the
-// three-part split's 60 kB of LDS allow two).  WPE changes the register
+// spilled registers; the FP32 and BF16 forms fit three either way, and so
+// does the three-part split's 48 kB of LDS).  WPE changes the register
 // budget only, not the arithmetic: the autotuner may pick 2.
 template <int FORM, int WPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
@@ -192,19 +203,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
                         acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(ra[a], rb[b], acc[a][b], 0, 0, 0);
             }
         } else {
+            const int sw = bf16_swz(li);  // rows wi*64 + a*32 + li: the swizzle depends on li only
 #pragma unroll
             for (int ks = 0; ks < BK; ks += 16) {
                 bf16x8 ah[2], al[2], am[2], bh[2], bl[2], bm[2];
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
-                    const int off = (wi * 64 + a * 32 + li) * PH + ks + lk * 8;
+                    const int off = (wi * 64 + a * 32 + li) * PH + ((((ks >> 3) + lk) ^ sw) << 3);
                     ah[a] = *reinterpret_cast<const bf16x8*>(sBh + off);
                     if constexpr (FORM >= 2) al[a] = *reinterpret_cast<const bf16x8*>(sBl + off);
                     if constexpr (FORM == 3) am[a] = *reinterpret_cast<const bf16x8*>(sBm + off);
                 }
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    const int off = (wj * 64 + b * 32 + li) * PH + ks + lk * 8;
+                    const int off = (wj * 64 + b * 32 + li) * PH + ((((ks >> 3) + lk) ^ sw) << 3);
                     bh[b] = *reinterpret_cast<const bf16x8*>(sAh + off);
                     if constexpr (FORM >= 2) bl[b] = *reinterpret_cast<const bf16x8*>(sAl + off);
                     if constexpr (FORM == 3) bm[b] = *reinterpret_cast<const bf16x8*>(sAm + off);
