@@ -83,6 +83,21 @@ __device__ __forceinline__ void fetch_tile(float4* v, const float* __restrict__ 
 // BF16 forms: the tile converted once, at the store to LDS (hi = bf16(x); for
 // the split forms lo = bf16(x - hi), or mid = bf16(x - hi) and
 // lo = bf16(x - hi - mid); the differences are exact in FP32).
+// Pairs are rounded together (one packed RNE conversion per pair) and the
+// two BF16 values are widened back by a shift and a mask of the packed word,
+// so the word that goes to LDS is the one the residual was formed from.
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2_t;
+typedef __attribute__((ext_vector_type(2))) float f32x2_t;
+
+__device__ __forceinline__ bf16x2_t bf16_pair(float a, float b, float& fa, float& fb) {
+    const f32x2_t x = {a, b};
+    const bf16x2_t p = __builtin_convertvector(x, bf16x2_t);
+    const unsigned w = __builtin_bit_cast(unsigned, p);
+    fa = __uint_as_float(w << 16);
+    fb = __uint_as_float(w & 0xFFFF0000u);
+    return p;
+}
+
 template <int FORM>
 __device__ __forceinline__ void put_tile_bf16(__bf16* __restrict__ sh, __bf16* __restrict__ sl,
                                               __bf16* __restrict__ sm, const float4* v) {
@@ -93,15 +108,27 @@ __device__ __forceinline__ void put_tile_bf16(__bf16* __restrict__ sh, __bf16* _
         const float x[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
         bf16x4 h, l, m;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const __bf16 he = (__bf16)x[e];
-            h[e] = he;
-            const float r1 = x[e] - (float)he;
-            if constexpr (FORM == 2) l[e] = (__bf16)r1;
+        for (int e = 0; e < 4; e += 2) {
+            float h0, h1;
+            const bf16x2_t hp = bf16_pair(x[e], x[e + 1], h0, h1);
+            h[e] = hp.x;
+            h[e + 1] = hp.y;
+            const float r0 = x[e] - h0, r1 = x[e + 1] - h1;
+            if constexpr (FORM == 2) {
+                const f32x2_t rr = {r0, r1};
+                const bf16x2_t lp = __builtin_convertvector(rr, bf16x2_t);
+                l[e] = lp.x;
+                l[e + 1] = lp.y;
+            }
             if constexpr (FORM == 3) {
-                const __bf16 me = (__bf16)r1;
-                m[e] = me;
-                l[e] = (__bf16)(r1 - (float)me);
+                float m0, m1;
+                const bf16x2_t mp = bf16_pair(r0, r1, m0, m1);
+                m[e] = mp.x;
+                m[e + 1] = mp.y;
+                const f32x2_t rr = {r0 - m0, r1 - m1};
+                const bf16x2_t lp = __builtin_convertvector(rr, bf16x2_t);
+                l[e] = lp.x;
+                l[e + 1] = lp.y;
             }
         }
         // row q / 8, unit (q % 8) / 2, half q % 2 of the unit
