@@ -133,6 +133,33 @@ __device__ __forceinline__ void lds_put(float* p, const float* in) {
     }
 }
 
+// max_m |a[m]| with the magnitudes taken by the max instructions' abs source
+// modifiers: one v_max3_f32 per two further values.  (fmaxf(fabsf(.), .)
+// compiles to a canonicalising max(x, x) per operand first.)  Same value as
+// the fmaxf chain: maxNum returns one of its operands, a NaN only when all
+// are NaN, and no sNaN reaches it (every operand is a computed value).
+template <int KPL>
+__device__ __forceinline__ float abs_max(const float* a) {
+    if constexpr (KPL == 1) {
+        return fabsf(a[0]);
+    } else if constexpr (KPL == 2) {
+        float t;
+        asm("v_max_f32_e64 %0, |%1|, |%2|" : "=v"(t) : "v"(a[0]), "v"(a[1]));
+        return t;
+    } else {
+        float t;
+        asm("v_max3_f32 %0, |%1|, |%2|, |%3|" : "=v"(t) : "v"(a[0]), "v"(a[1]), "v"(a[2]));
+#pragma unroll
+        for (int m = 3; m < KPL; m += 2) {
+            if (m + 1 < KPL)
+                asm("v_max3_f32 %0, %1, |%2|, |%3|" : "=v"(t) : "v"(t), "v"(a[m]), "v"(a[m + 1]));
+            else
+                asm("v_max_f32_e64 %0, %1, |%2|" : "=v"(t) : "v"(t), "v"(a[m]));
+        }
+        return t;
+    }
+}
+
 template <int KPL, int TT, int W, bool EPS>
 __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_qform_kernel(OmpArgs a) {
     // L packed column by column (lower triangle): L(r, c), r >= c, at Lp(r, c)
@@ -219,9 +246,7 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             // (maxabs, S/OMP.cu:201-213; lanes own increasing atom ranges).
             // |alpha| >= 0, so its bit pattern orders like a signed int; the
             // magnitudes come from the max instructions' abs modifiers
-            float mf = fabsf(al[0]);
-#pragma unroll
-            for (int m = 1; m < KPL; ++m) mf = fmaxf(mf, fabsf(al[m]));
+            const float mf = abs_max<KPL>(al);
             const int bv = __float_as_int(mf);
             const int M = wave_max_i32_bcast(bv);
             const int wl = __builtin_ctzll(__builtin_amdgcn_ballot_w64(bv == M));
@@ -383,7 +408,8 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
             } else if (i < QT) {
                 lds_put<KPL>(&Qs[v][i > QR ? i - QR : 0][lane * KPL], acc);
             }
-            if (lane == i) sp.ind = pos;
+            // support slot i <- pos (lane i), one v_writelane
+            asm("v_writelane_b32 %0, %1, %2" : "+v"(sp.ind) : "s"(pos), "n"(i < 64 ? i : 0));
             cnt = i + 1;
         }
     }
