@@ -52,16 +52,33 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
 
 // DPP row permutation of a float (all 64 lanes active; every source lane valid
-// for the controls used here).
+// for the controls used here, so with all rows and banks enabled no lane keeps
+// the old value and bound_ctrl lets the compiler leave it unset).
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 // DPP row permutation of an int (same controls as dpp_f).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
+}
+
+// v += row_bcast of v into the rows of ROWMASK (0xA: rows 1 and 3 take the
+// last lane of rows 0 and 2, row_bcast:15; 0xC: rows 2 and 3 take lane 31,
+// row_bcast:31); the other rows keep v.  One v_add_f32_dpp: the rows it
+// leaves out keep the destination, which is v (the compiler does not fold a
+// -0.0 identity into the add and materialises it instead).  The s_nop gives
+// the two wait states a DPP read needs after the VALU write of v.
+template <int BCAST>
+__device__ __forceinline__ float add_row_bcast(float v) {
+    static_assert(BCAST == 15 || BCAST == 31, "row_bcast:15 or row_bcast:31");
+    if constexpr (BCAST == 15)
+        asm("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(v));
+    else
+        asm("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(v));
+    return v;
 }
 
 // Signed-int wave max (DPP within rows, v_readlane across the four rows);
@@ -93,8 +110,8 @@ __device__ __forceinline__ int wave_max_i32_bcast(int v) {
 }
 
 // lanes_sum_dpp with the row sums combined by row_bcast:15 / row_bcast:31
-// (the same additions as lanes_sum_dpp, one v_readlane; -0.0, the identity
-// of the addition, in the rows left out)
+// (the same additions as lanes_sum_dpp, one v_readlane; the rows left out
+// keep their value)
 template <int NR>
 __device__ __forceinline__ float lanes_sum_bcast(float v) {
     v += dpp_f<0xB1>(v);
@@ -102,9 +119,9 @@ __device__ __forceinline__ float lanes_sum_bcast(float v) {
     v += dpp_f<0x141>(v);
     v += dpp_f<0x140>(v);
     if constexpr (NR == 1) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    v += __int_as_float(__builtin_amdgcn_update_dpp((int)0x80000000, __float_as_int(v), 0x142, 0xA, 0xF, false));
+    v = add_row_bcast<15>(v);
     if constexpr (NR == 2) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
-    v += __int_as_float(__builtin_amdgcn_update_dpp((int)0x80000000, __float_as_int(v), 0x143, 0xC, 0xF, false));
+    v = add_row_bcast<31>(v);
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
