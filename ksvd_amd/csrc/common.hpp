@@ -262,7 +262,7 @@ hipError_t launch_compact_to_slotmap(const int32_t* idx, const float* val, const
                                      float* sel, float* c, hipStream_t st);
 
 // Dictionary update / error / clear (dict_update.hip)
-constexpr int kDuBlocks = 512;      // default workgroups of the per-atom phase kernels (one block partial each)
+constexpr int kDuBlocks = 768;      // default workgroups of the per-atom phase kernels (one block partial each)
 constexpr int kDuBlocksMax = 1024;  // the most the scratch (apart, bpart) is sized for
 struct DuWork {
     int n, K, N, T, max_signals, use_thresh;
