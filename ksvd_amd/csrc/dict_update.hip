@@ -482,7 +482,7 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
 // workgroups 0 .. n-1, one element each), and du_tail_kernel does it for the
 // last atom of the sweep; nothing reads d_p in between.
 constexpr int DU_WAVES = 8;
-constexpr int FQ = 4;   // rows per batch in du_ba_kernel, divided by the float4 chunks per row
+constexpr int FQ = 2;   // rows per batch in du_ba_kernel, divided by the float4 chunks per row (59 VGPRs at n <= 256)
 constexpr int FQA = 8;  // residual rows in flight per wave in du_a_kernel (phase A of the first atom)
 constexpr int DU_RB = 16;  // workgroups of the phase A reduction
 
