@@ -703,9 +703,20 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* _
     const int cg = c0 + cl;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     if (cg < c1) {
-        for (int b = sl; b < G; b += S) {
-            const float4 v4 = *reinterpret_cast<const float4*>(apart + (long)b * np4 + 4 * cg);
-            acc = make_float4(acc.x + v4.x, acc.y + v4.y, acc.z + v4.z, acc.w + v4.w);
+        // RU partials in flight per thread (the partials come from other
+        // XCDs' L2s, so each load is a long round trip), added in order b
+        constexpr int RU = 8;
+        for (int b0 = sl; b0 < G; b0 += RU * S) {
+            float4 v4[RU];
+#pragma unroll
+            for (int u = 0; u < RU; ++u) {
+                const int b = b0 + u * S;
+                v4[u] = (b < G) ? *reinterpret_cast<const float4*>(apart + (long)b * np4 + 4 * cg)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < RU; ++u)
+                if (b0 + u * S < G) acc = make_float4(acc.x + v4[u].x, acc.y + v4[u].y, acc.z + v4[u].z, acc.w + v4[u].w);
         }
     }
     red[threadIdx.x] = acc;
