@@ -1023,9 +1023,19 @@ __global__ __launch_bounds__(256) void error_from_residual_kernel(const float* _
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double acc = 0.0;
     for (long s = (long)blockIdx.x * 4 + w; s < N; s += (long)gridDim.x * 4) {
+        // four loads of the row in flight, summed in the order t = lane, lane + 64, ...
+        const float* __restrict__ r_s = R + s * n;
         float e = 0.f;
-        for (int t = lane; t < n; t += 64) {
-            const float r = R[s * n + t];
+        int t = lane;
+        for (; t + 192 < n; t += 256) {
+            const float r0 = r_s[t], r1 = r_s[t + 64], r2 = r_s[t + 128], r3 = r_s[t + 192];
+            e += r0 * r0;
+            e += r1 * r1;
+            e += r2 * r2;
+            e += r3 * r3;
+        }
+        for (; t < n; t += 64) {
+            const float r = r_s[t];
             e += r * r;
         }
         e = wave_sum(e);
@@ -1050,19 +1060,36 @@ __global__ __launch_bounds__(1024) void error_finish_kernel(const double* __rest
 // usecount_k = #{s : gamma_ks^2 > 1e-7}  (use_count_kernel, S/KSVD.cu:403-415):
 // per-block LDS histogram over a grid-stride range of compact entries (thread
 // = entry s*T + j, so the idx/val loads are coalesced), then one global add
-// per atom and block (integer counts: order independent).
+// per atom and block (integer counts: order independent).  The stride is the
+// largest multiple of T the grid covers, so each thread keeps one slot j and
+// steps its signal by stride / T (no division per entry; N*T < 2^31).
 __global__ __launch_bounds__(256) void usecount_kernel(const int32_t* __restrict__ idx, const float* __restrict__ val,
                                                        const int32_t* __restrict__ cnt, int N, int T, int K,
                                                        int32_t* __restrict__ usecount) {
     extern __shared__ int32_t hist[];
     for (int k = threadIdx.x; k < K; k += blockDim.x) hist[k] = 0;
     __syncthreads();
-    const long NT = (long)N * T;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < NT; e += (long)gridDim.x * blockDim.x) {
-        const long s = e / T;
-        if ((int)(e - s * T) < cnt[s]) {
-            const float v = val[e];
-            if (v * v > 1e-7f) atomicAdd(&hist[idx[e]], 1);
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sstep = (int)(gridDim.x * blockDim.x) / T;  // signals per sweep of the grid
+    if (gt < sstep * T) {
+        const int j = gt % T;
+        // four signals per round: their count, coefficient and atom loads all
+        // in flight together (slots past the count are in bounds, unused)
+        constexpr int SU = 4;
+        for (int s = gt / T; s < N; s += SU * sstep) {
+            int c[SU], k[SU];
+            float v[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) {
+                const int su = s + u * sstep;
+                const bool in = su < N;
+                c[u] = in ? cnt[su] : 0;
+                v[u] = in ? val[su * T + j] : 0.f;
+                k[u] = in ? idx[su * T + j] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; ++u)
+                if (j < c[u] && v[u] * v[u] > 1e-7f) atomicAdd(&hist[k[u]], 1);
         }
     }
     __syncthreads();
