@@ -506,7 +506,9 @@ int autotune(ksvd_handle* h, const float* D, const float* X, const float* eps2, 
         HIPTRY(h, hipStreamSynchronize(h->stream), "perm upload sync");
     }
     double bestd = 0.0;
-    for (int blocks : {512, 256, 768, 1024}) {
+    // the built-in grid first (it is the reference time), then the others
+    for (int blocks : {ksvd::kDuBlocks, 256, 512, 768, 1024}) {
+        if (blocks == ksvd::kDuBlocks && bestd > 0.0) continue;
         h->du.blocks = blocks;
         double ms = 1e30;
         for (int rep = 0; rep < 2; ++rep) {  // the first sweep warms up
@@ -532,8 +534,8 @@ int autotune(ksvd_handle* h, const float* D, const float* X, const float* eps2, 
             if (rep > 0 && one < ms) ms = one;
         }
         note("du blocks=%d %.4f ms; ", blocks, ms);
-        if (blocks == 512) best.default_du_ms = ms;
-        if (blocks == 512 || ms < bestd) { bestd = ms; best.du_blocks = blocks; }
+        if (blocks == ksvd::kDuBlocks) best.default_du_ms = ms;
+        if (bestd == 0.0 || ms < bestd) { bestd = ms; best.du_blocks = blocks; }
     }
     best.du_ms = bestd;
     h->du.blocks = best.du_blocks;
