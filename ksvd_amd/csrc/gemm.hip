@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int jl = wi * 64 + a * 32 + 8 * (r >> 2) + 4 * lk + (r & 3);
-                if (full || (ml < nA && jl < nB)) Ct[jl * colsA + ml] = acc[a][b][r];
+                if (full || (ml < nA && jl < nB)) __builtin_nontemporal_store(acc[a][b][r], Ct + (jl * colsA + ml));
             }
         }
 }
