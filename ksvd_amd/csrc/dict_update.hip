@@ -503,6 +503,17 @@ __device__ __forceinline__ void st4(float* __restrict__ p, int t, int n, bool ve
     if (t + 3 < n) p[t + 3] = v.w;
 }
 
+// st4 with a streaming (non-temporal) store: a residual row written by phase
+// B is not read again for about T / K of a sweep, so it should not displace
+// the rows phase A just read, which the next launch's phase B reads again.
+__device__ __forceinline__ void st4_nt(float* __restrict__ p, int t, int n, bool vec, float4 v) {
+    if (vec) {
+        if (t < n) __builtin_nontemporal_store(f32x4{v.x, v.y, v.z, v.w}, reinterpret_cast<f32x4*>(p + t));
+        return;
+    }
+    st4(p, t, n, vec, v);
+}
+
 __device__ __forceinline__ float4 f4fma(float a, float4 x, float4 y) {
     return make_float4(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y), fmaf(a, x.z, y.z), fmaf(a, x.w, y.w));
 }
@@ -868,7 +879,7 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
 #pragma unroll
                     for (int c = 0; c < NCH; ++c) {
                         const float4 rn = f4fma(-dot, dn[c], e[u][c]);
-                        st4(R + s * n, 256 * c + 4 * lane, n, vec, rn);
+                        st4_nt(R + s * n, 256 * c + 4 * lane, n, vec, rn);
                         if (gq != 0.f) acc[c] = f4fma(gq, f4fma(gq, dq[c], rn), acc[c]);
                     }
                     if (lane == 0) val[readlane(posb, k + u)] = dot;
