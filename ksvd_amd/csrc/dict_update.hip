@@ -503,6 +503,17 @@ __device__ __forceinline__ void st4(float* __restrict__ p, int t, int n, bool ve
     if (t + 3 < n) p[t + 3] = v.w;
 }
 
+// ld4 with a streaming (non-temporal) load: phase B reads a row for the last
+// time this stretch of the sweep
+__device__ __forceinline__ float4 ld4_nt(const float* __restrict__ p, int t, int n, bool vec) {
+    if (vec) {
+        if (t >= n) return make_float4(0.f, 0.f, 0.f, 0.f);
+        const f32x4 u = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p + t));
+        return make_float4(u[0], u[1], u[2], u[3]);
+    }
+    return ld4(p, t, n, vec);
+}
+
 // st4 with a streaming (non-temporal) store: a residual row written by phase
 // B is not read again for about T / K of a sweep, so it should not displace
 // the rows phase A just read, which the next launch's phase B reads again.
@@ -851,7 +862,7 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
                 const float g = okb ? readlane(gb, k + u) : 0.f;
 #pragma unroll
                 for (int c = 0; c < NCH; ++c)  // e = r_s + d_p gamma_s
-                    e[u][c] = okb ? f4fma(g, dv[c], ld4(R + s * n, 256 * c + 4 * lane, n, vec))
+                    e[u][c] = okb ? f4fma(g, dv[c], ld4_nt(R + s * n, 256 * c + 4 * lane, n, vec))
                                   : make_float4(0.f, 0.f, 0.f, 0.f);
                 const bool oka = k + u < nka && !readlane(sha, k + u < nka ? k + u : 0);
                 const long sa = readlane(sla, oka ? k + u : 0);
