@@ -930,7 +930,9 @@ __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const 
 // R = X - D Gamma for n <= 1024: one wave per signal, each lane owns 4
 // consecutive rows per 256 (float4), RES_COLS dictionary columns in flight
 // (8 measured slower than 4 at n = 256); the signal's own row of X is fetched
-// first so its latency overlaps the gathers.
+// first so its latency overlaps the gathers.  X is read and R written with
+// streaming accesses (each touched once here), so that L2 keeps the
+// dictionary columns the gathers share.
 template <int NCH>
 __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restrict__ X, const float* __restrict__ D,
                                                            const int32_t* __restrict__ idx,
@@ -949,7 +951,7 @@ __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restri
         float4 x4[NCH], xh[NCH];
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
-            x4[ch] = ld4(X + s * n, 256 * ch + 4 * lane, n, vec);
+            x4[ch] = ld4_nt(X + s * n, 256 * ch + 4 * lane, n, vec);
             xh[ch] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         for (int j = 0; j < c; j += RES_COLS) {
@@ -971,8 +973,8 @@ __global__ __launch_bounds__(256) void du_residual4_kernel(const float* __restri
         }
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch)
-            st4(R + s * n, 256 * ch + 4 * lane, n, vec,
-                make_float4(x4[ch].x - xh[ch].x, x4[ch].y - xh[ch].y, x4[ch].z - xh[ch].z, x4[ch].w - xh[ch].w));
+            st4_nt(R + s * n, 256 * ch + 4 * lane, n, vec,
+                   make_float4(x4[ch].x - xh[ch].x, x4[ch].y - xh[ch].y, x4[ch].z - xh[ch].z, x4[ch].w - xh[ch].w));
     }
 }
 
@@ -1050,7 +1052,8 @@ __global__ __launch_bounds__(256) void error_from_residual_kernel(const float* _
         float e = 0.f;
         int t = lane;
         for (; t + 192 < n; t += 256) {
-            const float r0 = r_s[t], r1 = r_s[t + 64], r2 = r_s[t + 128], r3 = r_s[t + 192];
+            const float r0 = __builtin_nontemporal_load(r_s + t), r1 = __builtin_nontemporal_load(r_s + t + 64),
+                        r2 = __builtin_nontemporal_load(r_s + t + 128), r3 = __builtin_nontemporal_load(r_s + t + 192);
             e += r0 * r0;
             e += r1 * r1;
             e += r2 * r2;
@@ -1105,9 +1108,9 @@ __global__ __launch_bounds__(256) void usecount_kernel(const int32_t* __restrict
             for (int u = 0; u < SU; ++u) {
                 const int su = s + u * sstep;
                 const bool in = su < N;
-                c[u] = in ? cnt[su] : 0;
-                v[u] = in ? val[su * T + j] : 0.f;
-                k[u] = in ? idx[su * T + j] : 0;
+                c[u] = in ? __builtin_nontemporal_load(cnt + su) : 0;
+                v[u] = in ? __builtin_nontemporal_load(val + su * T + j) : 0.f;
+                k[u] = in ? __builtin_nontemporal_load(idx + su * T + j) : 0;
             }
 #pragma unroll
             for (int u = 0; u < SU; ++u)
