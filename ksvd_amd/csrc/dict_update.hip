@@ -553,22 +553,23 @@ __device__ __forceinline__ void du_commit_column(int jj, const uint32_t* __restr
 }
 
 // The unused-atom special case (S/DictionaryUpdate.cu:238-302, SCupdate
-// :406-427) for atom p, run by every block of the launch: errors ||r_b||^2 of
-// the first min(N - used, M) live unused-list entries, arg-max (lowest index
-// on ties) through block partials and a last-block ticket, then
-// d_p = x / ||x||, swap-remove, replaced, bitmap.
-__device__ void du_special_case(int p, const float* __restrict__ X, float* __restrict__ D,
-                                const float* __restrict__ R, int32_t* __restrict__ unused,
-                                int32_t* __restrict__ usc, int32_t* __restrict__ bitmap,
-                                int32_t* __restrict__ replaced, int n, int N, int M, float* __restrict__ bpart,
-                                int32_t* __restrict__ ticket, float* rv, int* ri, int* flag_sh, float* wred) {
+// :406-427) for atom p, in two parts.  du_special_partial, run by every block
+// of the launch that meets the unused atom: errors ||r_b||^2 of the first
+// min(N - used, M) live unused-list entries and each block's arg-max (lowest
+// index on ties) in bpart.  du_special_finish, run by one workgroup of the
+// next launch (the kernel boundary publishes bpart; a last-block ticket would
+// need a device-scope fence in every block, ~0.1 ms per atom here): the
+// block maxima in block order, then d_p = x / ||x||, swap-remove, replaced,
+// bitmap.
+__device__ void du_special_partial(const float* __restrict__ R, const int32_t* __restrict__ unused,
+                                   const int32_t* __restrict__ usc, int n, int N, int M, float* __restrict__ bpart,
+                                   float* rv, int* ri) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int G = gridDim.x;
     const int counter = *usc;
     const int lim = min(M, N - counter);
     // every signal already replaced an atom this sweep: no candidate, the atom
-    // stays as it is (the reference would read before its unused list); the
-    // same value in every block, so no block takes the ticket below
+    // stays as it is (the reference would read before its unused list)
     if (lim <= 0) return;
     float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
     int bi = 0;
@@ -592,7 +593,17 @@ __device__ void du_special_case(int p, const float* __restrict__ X, float* __res
         bpart[2 * blockIdx.x] = v;
         bpart[2 * blockIdx.x + 1] = __int_as_float(i);
     }
-    if (!last_block(ticket, flag_sh)) return;
+}
+
+// G: the grid of the launch that ran du_special_partial.  Uniform control
+// flow (one whole workgroup, any size up to 1024).
+__device__ void du_special_finish(int p, int G, const float* __restrict__ X, float* __restrict__ D,
+                                  int32_t* __restrict__ unused, int32_t* __restrict__ usc,
+                                  int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n, int N, int M,
+                                  const float* __restrict__ bpart, int* ri, float* wred) {
+    const int counter = *usc;
+    const int lim = min(M, N - counter);
+    if (lim <= 0) return;  // the partial step wrote nothing either
     if (threadIdx.x == 0) {
         float v = bpart[0];
         int i = __float_as_int(bpart[1]);
@@ -639,8 +650,6 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a
     __shared__ float rv[DU_WAVES];
     __shared__ int ri[DU_WAVES];
-    __shared__ int flag_sh;
-    __shared__ float wred[16];
     const int p = (int)perm[jj];
     const int Np = count[p];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -648,7 +657,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
     const bool vec = (n & 3) == 0;
     if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, abuf, anorm, n);
     if (Np == 0) {
-        du_special_case(p, X, D, R, unused, usc, bitmap, replaced, n, N, M, bpart, ticket, rv, ri, &flag_sh, wred);
+        du_special_partial(R, unused, usc, n, N, M, bpart, rv, ri);  // finished by du_a_reduce_kernel(0)
         return;
     }
     // a = sum_{s in I_p} gamma_s (r_s + d_p gamma_s)
@@ -712,8 +721,21 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
 // no dependent load.
 __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* __restrict__ apart,
                                                           float* __restrict__ abuf, float* __restrict__ anorm, int n,
-                                                          int G) {
+                                                          int G, int fpos, const int4* __restrict__ sweep,
+                                                          const float* __restrict__ X, float* __restrict__ D,
+                                                          int32_t* __restrict__ unused, int32_t* __restrict__ usc,
+                                                          int32_t* __restrict__ bitmap,
+                                                          int32_t* __restrict__ replaced, int N, int M,
+                                                          const float* __restrict__ bpart) {
     __shared__ float4 red[512];
+    __shared__ int fri[1];
+    __shared__ float fwred[16];
+    // workgroup 0 first finishes the special case of sweep position fpos
+    // when its atom is unused (begun by the previous launch)
+    if (fpos >= 0 && blockIdx.x == 0) {
+        const int4 d = sweep[2 * fpos];
+        if (d.y == 0) du_special_finish(d.x, G, X, D, unused, usc, bitmap, replaced, n, N, M, bpart, fri, fwred);
+    }
     const int np4 = (n + 3) & ~3;
     const int ng4 = np4 / 4;
     const int r = blockIdx.x;
@@ -799,8 +821,6 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a of the next atom
     __shared__ float rv[DU_WAVES];
     __shared__ int ri[DU_WAVES];
-    __shared__ int flag_sh;
-    __shared__ float wred[16];
     // this position's atom p and the next one q, their list lengths and offsets
     const int4 d0 = sweep[2 * jj], d1 = sweep[2 * jj + 1];
     const int p = d0.x, Np = d0.y, q = d0.w, Nq = d1.x;
@@ -917,14 +937,23 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
             apart[(long)blockIdx.x * np4 + t] = sacc;
         }
     }
-    if (Np == 0 && jj > 0) du_special_case(p, X, D, R, unused, usc, bitmap, replaced, n, N, M, bpart, ticket, rv, ri,
-                                           &flag_sh, wred);
+    // an unused atom: finished by the next launch (du_a_reduce_kernel(jj + 1), or du_tail_kernel)
+    if (Np == 0 && jj > 0) du_special_partial(R, unused, usc, n, N, M, bpart, rv, ri);
 }
 
+// The last sweep position: its new column, or (an unused atom, jj > 0) the
+// special case begun by du_ba_kernel(jj).
 __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const int32_t* __restrict__ count,
                                float* __restrict__ D, const float* __restrict__ abuf,
-                               const float* __restrict__ anorm, int n) {
+                               const float* __restrict__ anorm, int n, int G, const float* __restrict__ X,
+                               int32_t* __restrict__ unused, int32_t* __restrict__ usc,
+                               int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int N, int M,
+                               const float* __restrict__ bpart) {
+    __shared__ int fri[1];
+    __shared__ float fwred[16];
     du_commit_column(jj, perm, count, D, abuf, anorm, n);
+    const int p = (int)perm[jj];
+    if (jj > 0 && count[p] == 0) du_special_finish(p, G, X, D, unused, usc, bitmap, replaced, n, N, M, bpart, fri, fwred);
 }
 
 // R = X - D Gamma for n <= 1024: one wave per signal, each lane owns 4
@@ -1004,16 +1033,19 @@ void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hip
     hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(nb), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
                        w.list_rec, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
                        M, w.apart, w.bpart, abuf, anorm, w.tickets);
-    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.apart, abuf, anorm, w.n, nb);
+    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.apart, abuf, anorm, w.n, nb, 0,
+                       w.sweep, X, D, w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart);
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(nb), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
                            w.ptr, w.list_rec, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
                            w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets, w.sweep);
         if (jj + 1 < w.K)
             hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.apart, abuf, anorm, w.n,
-                               nb);
+                               nb, jj >= 1 ? jj : -1, w.sweep, X, D, w.unused, w.us_counter, w.bitmap, w.replaced,
+                               w.N, M, w.bpart);
     }
-    hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n);
+    hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n,
+                       nb, X, w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart);
 }
 
 // ---- error: S/KSVD.cu:235-262, S/ReductionKernels.cu:257-322 ---------------
