@@ -596,7 +596,8 @@ __device__ void du_special_partial(const float* __restrict__ R, const int32_t* _
 }
 
 // G: the grid of the launch that ran du_special_partial.  Uniform control
-// flow (one whole workgroup, any size up to 1024).
+// flow (one whole workgroup, any size up to 1024); ri and wred hold 16
+// entries.
 __device__ void du_special_finish(int p, int G, const float* __restrict__ X, float* __restrict__ D,
                                   int32_t* __restrict__ unused, int32_t* __restrict__ usc,
                                   int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n, int N, int M,
@@ -604,15 +605,31 @@ __device__ void du_special_finish(int p, int G, const float* __restrict__ X, flo
     const int counter = *usc;
     const int lim = min(M, N - counter);
     if (lim <= 0) return;  // the partial step wrote nothing either
+    // the largest block maximum, lowest entry among equal ones (the order of
+    // the comparisons does not matter: every block maximum is >= 0 and the
+    // entries of different blocks differ), in parallel over the workgroup
+    float v = -1.f;
+    int i = 0x7fffffff;
+    for (int q = threadIdx.x; q < G; q += blockDim.x) {
+        const float vq = bpart[2 * q];
+        const int iq = __float_as_int(bpart[2 * q + 1]);
+        if (vq > v || (vq == v && iq < i)) { v = vq; i = iq; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float vo = __shfl_xor(v, off);
+        const int io = __shfl_xor(i, off);
+        if (vo > v || (vo == v && io < i)) { v = vo; i = io; }
+    }
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) { wred[w] = v; ri[w] = i; }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        float v = bpart[0];
-        int i = __float_as_int(bpart[1]);
-        for (int q = 1; q < G; ++q) {
-            const float vq = bpart[2 * q];
-            const int iq = __float_as_int(bpart[2 * q + 1]);
-            if (vq > v || (vq == v && iq < i)) { v = vq; i = iq; }
-        }
-        ri[0] = i;
+        float vb = wred[0];
+        int ib = ri[0];
+        for (int q = 1; q < nw; ++q)
+            if (wred[q] > vb || (wred[q] == vb && ri[q] < ib)) { vb = wred[q]; ib = ri[q]; }
+        ri[0] = ib;
     }
     __syncthreads();
     const int best = ri[0];
@@ -728,7 +745,7 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* _
                                                           int32_t* __restrict__ replaced, int N, int M,
                                                           const float* __restrict__ bpart) {
     __shared__ float4 red[512];
-    __shared__ int fri[1];
+    __shared__ int fri[16];
     __shared__ float fwred[16];
     // workgroup 0 first finishes the special case of sweep position fpos
     // when its atom is unused (begun by the previous launch)
@@ -949,7 +966,7 @@ __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const 
                                int32_t* __restrict__ unused, int32_t* __restrict__ usc,
                                int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int N, int M,
                                const float* __restrict__ bpart) {
-    __shared__ int fri[1];
+    __shared__ int fri[16];
     __shared__ float fwred[16];
     du_commit_column(jj, perm, count, D, abuf, anorm, n);
     const int p = (int)perm[jj];
