@@ -157,10 +157,6 @@ int check_cfg(ksvd_handle* h, const ksvd_config* c) {
     return KSVD_OK;
 }
 
-__global__ void zero_i32_kernel(int32_t* p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
-}
-
 int ensure_du(ksvd_handle* h) {
     if (h->du_ready) return KSVD_OK;
     ksvd::DuWork& w = h->du;
@@ -182,10 +178,6 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.apart, (size_t)ksvd::kDuBlocksMax * ((c.n + 3) & ~3), "apart"));
     TRY(alloc(h, &w.bpart, (size_t)2 * ksvd::kDuBlocksMax, "bpart"));
     TRY(alloc(h, &w.dnew, (size_t)2 * c.n + 32, "dnew"));
-    TRY(alloc(h, &w.scal, 8, "scal"));
-    TRY(alloc(h, &w.tickets, 4, "tickets"));
-    hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, h->stream, w.tickets, 4);
-    HIPTRY(h, hipGetLastError(), "tickets");
     TRY(alloc(h, &w.unused, c.N, "unused"));
     TRY(alloc(h, &w.us_counter, 1, "us_counter"));
     TRY(alloc(h, &w.bitmap, c.N, "bitmap"));

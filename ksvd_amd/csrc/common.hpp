@@ -280,8 +280,6 @@ struct DuWork {
     float* apart;        // blocks x (n rounded up to 4) partial a
     float* bpart;        // blocks x 2 partial arg-max (special case)
     float* dnew;         // 2 x n + 2 x 16: a and its partial sums of squares, by sweep-position parity
-    float* scal;         // 8 scalars (norm etc.)
-    int32_t* tickets;    // 4 last-block-done counters (zero between launches)
     int32_t* unused;     // N
     int32_t* us_counter; // 1
     int32_t* bitmap;     // N

@@ -255,35 +255,27 @@ __global__ __launch_bounds__(256) void du_residual_kernel(const float* __restric
     }
 }
 
-// Last-block-done: returns true in exactly one block, after every block of
-// the grid has made its global writes visible.  The last block re-arms the
-// ticket, so the kernel can be launched again (and replayed in a graph).
-__device__ __forceinline__ bool last_block(int32_t* ticket, int* flag_sh) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int old = atomicAdd(ticket, 1);
-        *flag_sh = (old == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    const bool last = *flag_sh != 0;
-    if (last) {
-        __threadfence();
-        if (threadIdx.x == 0) *ticket = 0;
-    }
-    return last;
-}
 
-// Per-atom phase A.
-//   N_p > 0:  a = sum_{s in I_p} gamma_s (r_s + d_p gamma_s)  ( = E_p gamma^T,
-//             S/Multiplication.cu:177-193 computes the same vector as
-//             Y_p - D v + d_p ||gamma||^2 );  the last block forms ||a|| and
-//             d_new = a / ||a|| (S/Multiplication.cu:221).
-//   N_p = 0:  the unused-atom special case (S/DictionaryUpdate.cu:238-302):
-//             errors ||r_b||^2 = ||x - D Gamma||^2 of the first
-//             min(N, max_signals) live unused-list entries, arg-max (lowest
-//             index on ties), d_p = x / ||x||, swap-remove, replaced, bitmap
-//             (SCupdate, S/DictionaryUpdate.cu:406-427).
+// (defined with the pipelined sweep below)
+__device__ void du_special_partial(const float* __restrict__ R, const int32_t* __restrict__ unused,
+                                   const int32_t* __restrict__ usc, int n, int N, int M, float* __restrict__ bpart,
+                                   float* rv, int* ri);
+__device__ __forceinline__ float du_anorm(int jj, const float* __restrict__ anorm);
+__device__ __forceinline__ void du_commit_column(int jj, const uint32_t* __restrict__ perm,
+                                                 const int32_t* __restrict__ count, float* __restrict__ D,
+                                                 const float* __restrict__ abuf, const float* __restrict__ anorm,
+                                                 int n);
+
+// Per-atom phase A (the sweep for n > 1024: per atom phase A, the reduction
+// du_a_reduce_kernel, phase B).  Block 0 first commits the new column of the
+// atom at position jj - 1.
+//   N_p > 0:  block partials of a = sum_{s in I_p} gamma_s (r_s + d_p gamma_s)
+//             ( = E_p gamma^T, S/Multiplication.cu:177-193 computes the same
+//             vector as Y_p - D v + d_p ||gamma||^2 ), rows of apart padded to
+//             n rounded up to 4; the reduction forms a and ||a||.
+//   N_p = 0:  the unused-atom special case (S/DictionaryUpdate.cu:238-302,
+//             SCupdate :406-427): block maxima of the errors, finished by the
+//             reduction's workgroup 0.
 __global__ __launch_bounds__(256) void du_phase_a_kernel(int jj, const uint32_t* __restrict__ perm,
                                                          const int32_t* __restrict__ count,
                                                          const int32_t* __restrict__ ptr,
@@ -293,75 +285,18 @@ __global__ __launch_bounds__(256) void du_phase_a_kernel(int jj, const uint32_t*
                                                          int32_t* __restrict__ unused, int32_t* __restrict__ usc,
                                                          int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced,
                                                          int n, int N, int T, int M, float* __restrict__ apart,
-                                                         float* __restrict__ bpart, float* __restrict__ dnew,
-                                                         float* __restrict__ scal, int32_t* __restrict__ ticket) {
+                                                         float* __restrict__ bpart, const float* __restrict__ abuf,
+                                                         const float* __restrict__ anorm) {
     extern __shared__ float sh[];  // 4 x n (per-wave partial a)
     __shared__ float rv[4];
     __shared__ int ri[4];
-    __shared__ int flag_sh;
-    __shared__ float wred[16];
     const int p = (int)perm[jj];
     const int Np = count[p];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int G = gridDim.x;
+    if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, abuf, anorm, n);
     if (Np == 0) {
-        const int counter = *usc;
-        const int lim = min(M, N - counter);
-        // every signal already replaced an atom this sweep: no candidate, the
-        // atom stays as it is (the reference would read before its unused list)
-        if (lim <= 0) return;
-        float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
-        int bi = 0;
-        for (int b = blockIdx.x * 4 + w; b < lim; b += G * 4) {
-            const long sig = unused[b];
-            float e = 0.f;
-            for (int t = lane; t < n; t += 64) {
-                const float r = R[sig * n + t];
-                e += r * r;
-            }
-            e = wave_sum(e);
-            if (e > bv) { bv = e; bi = b; }  // b ascending per wave: keeps the lowest on ties
-        }
-        if (lane == 0) { rv[w] = bv; ri[w] = bi; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float v = rv[0];
-            int i = ri[0];
-            for (int q = 1; q < 4; ++q)
-                if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
-            bpart[2 * blockIdx.x] = v;
-            bpart[2 * blockIdx.x + 1] = __int_as_float(i);
-        }
-        if (!last_block(ticket, &flag_sh)) return;
-        if (threadIdx.x == 0) {
-            float v = bpart[0];
-            int i = __float_as_int(bpart[1]);
-            for (int q = 1; q < G; ++q) {
-                const float vq = bpart[2 * q];
-                const int iq = __float_as_int(bpart[2 * q + 1]);
-                if (vq > v || (vq == v && iq < i)) { v = vq; i = iq; }
-            }
-            ri[0] = i;
-        }
-        __syncthreads();
-        const int best = ri[0];
-        const long sig = unused[best];
-        float sq = 0.f;
-        for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            const float x = X[sig * n + t];
-            sq += x * x;
-        }
-        sq = block_sum(sq, wred);
-        if (sq == 0.f) return;  // a zero signal (every candidate error 0): the atom stays
-        const float nrm = sqrtf(sq);
-        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unused[best] = unused[N - counter - 1];
-            *usc = counter + 1;
-            replaced[p] = 1;
-            bitmap[sig] = 0;
-        }
+        du_special_partial(R, unused, usc, n, N, M, bpart, rv, ri);
         return;
     }
     const int beg = ptr[p];
@@ -392,38 +327,34 @@ __global__ __launch_bounds__(256) void du_phase_a_kernel(int jj, const uint32_t*
         }
     }
     __syncthreads();
+    const int np4 = (n + 3) & ~3;
     for (int t = threadIdx.x; t < n; t += blockDim.x)
-        apart[(long)blockIdx.x * n + t] = ((sh[t] + sh[n + t]) + sh[2 * n + t]) + sh[3 * n + t];
-    if (!last_block(ticket, &flag_sh)) return;
-    float sq = 0.f;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        float a = 0.f;
-        for (int b = 0; b < G; ++b) a += apart[(long)b * n + t];
-        sh[t] = a;
-        sq += a * a;
-    }
-    sq = block_sum(sq, wred);
-    const float nrm = sqrtf(sq);
-    for (int t = threadIdx.x; t < n; t += blockDim.x) dnew[t] = sh[t] / nrm;
-    if (threadIdx.x == 0) scal[0] = nrm;
+        apart[(long)blockIdx.x * np4 + t] = ((sh[t] + sh[n + t]) + sh[2 * n + t]) + sh[3 * n + t];
 }
 
 // Per-atom phase B: gamma_new = d_new^T (r_s + d_p gamma_s) and the residual
 // update r_s <- r_s + d_p gamma_s - d_new gamma_new for every signal on the
 // list (S/Multiplication.cu:198-225 evaluates the same gamma_new as
-// (a^T x - u^T Gamma_s + u_p gamma_s) / ||a||); the last block writes
-// d_p = d_new (S/Multiplication.cu:221).
+// (a^T x - u^T Gamma_s + u_p gamma_s) / ||a||).  d_p = d_new
+// (S/Multiplication.cu:221) is written by the next phase A, or
+// du_tail_kernel after the last atom: every block of this launch reads d_p.
 __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t* __restrict__ perm,
                                                          const int32_t* __restrict__ count,
                                                          const int32_t* __restrict__ ptr,
                                                          const int32_t* __restrict__ list_pos, float* __restrict__ val,
                                                          float* __restrict__ D, float* __restrict__ R,
-                                                         const float* __restrict__ dnew, int n, int T,
-                                                         int32_t* __restrict__ ticket) {
-    __shared__ int flag_sh;
+                                                         const float* __restrict__ abuf,
+                                                         const float* __restrict__ anorm, int n, int T) {
+    extern __shared__ float sdn[];  // d_new = a / ||a|| (n floats)
     const int p = (int)perm[jj];
     const int Np = count[p];
     if (Np == 0) return;
+    {
+        const float nrm = du_anorm(jj, anorm);
+        for (int t = threadIdx.x; t < n; t += blockDim.x) sdn[t] = abuf[(long)(jj & 1) * n + t] / nrm;
+        __syncthreads();
+    }
+    const float* dnew = sdn;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int G = gridDim.x;
     const int beg = ptr[p];
@@ -468,8 +399,6 @@ __global__ __launch_bounds__(256) void du_phase_b_kernel(int jj, const uint32_t*
             if (lane == 0) val[pos] = gn;
         }
     }
-    if (!last_block(ticket, &flag_sh)) return;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = dnew[t];
 }
 
 // ---- per-atom phases, n <= 1024: batched, latency-hiding, pipelined ---------
@@ -556,15 +485,15 @@ __device__ __forceinline__ void du_commit_column(int jj, const uint32_t* __restr
 // :406-427) for atom p, in two parts.  du_special_partial, run by every block
 // of the launch that meets the unused atom: errors ||r_b||^2 of the first
 // min(N - used, M) live unused-list entries and each block's arg-max (lowest
-// index on ties) in bpart.  du_special_finish, run by one workgroup of the
-// next launch (the kernel boundary publishes bpart; a last-block ticket would
-// need a device-scope fence in every block, ~0.1 ms per atom here): the
-// block maxima in block order, then d_p = x / ||x||, swap-remove, replaced,
-// bitmap.
+// index on ties) in bpart (rv, ri: one entry per wave).  du_special_finish,
+// run by one workgroup of the next launch (the kernel boundary publishes
+// bpart; a last-block ticket would need a device-scope fence in every block,
+// ~0.1 ms per atom here): the largest block maximum, then d_p = x / ||x||,
+// swap-remove, replaced, bitmap.
 __device__ void du_special_partial(const float* __restrict__ R, const int32_t* __restrict__ unused,
                                    const int32_t* __restrict__ usc, int n, int N, int M, float* __restrict__ bpart,
                                    float* rv, int* ri) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int G = gridDim.x;
     const int counter = *usc;
     const int lim = min(M, N - counter);
@@ -573,7 +502,7 @@ __device__ void du_special_partial(const float* __restrict__ R, const int32_t* _
     if (lim <= 0) return;
     float bv = 0.f;  // the reference seeds its reduction with value 0 / index 0
     int bi = 0;
-    for (int b = blockIdx.x * DU_WAVES + w; b < lim; b += G * DU_WAVES) {
+    for (int b = blockIdx.x * nw + w; b < lim; b += G * nw) {
         const long sig = unused[b];
         float e = 0.f;
         for (int t = lane; t < n; t += 64) {
@@ -588,7 +517,7 @@ __device__ void du_special_partial(const float* __restrict__ R, const int32_t* _
     if (threadIdx.x == 0) {
         float v = rv[0];
         int i = ri[0];
-        for (int q = 1; q < DU_WAVES; ++q)
+        for (int q = 1; q < nw; ++q)
             if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
         bpart[2 * blockIdx.x] = v;
         bpart[2 * blockIdx.x + 1] = __int_as_float(i);
@@ -663,7 +592,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
                                                    int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
                                                    int N, int T, int M, float* __restrict__ apart,
                                                    float* __restrict__ bpart, const float* __restrict__ abuf,
-                                                   const float* __restrict__ anorm, int32_t* __restrict__ ticket) {
+                                                   const float* __restrict__ anorm) {
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a
     __shared__ float rv[DU_WAVES];
     __shared__ int ri[DU_WAVES];
@@ -833,7 +762,7 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
                                                     int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
                                                     int N, int T, int M, float* __restrict__ apart,
                                                     float* __restrict__ bpart, const float* __restrict__ abuf,
-                                                    const float* __restrict__ anorm, int32_t* __restrict__ ticket,
+                                                    const float* __restrict__ anorm,
                                                     const int4* __restrict__ sweep) {
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a of the next atom
     __shared__ float rv[DU_WAVES];
@@ -958,8 +887,8 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
     if (Np == 0 && jj > 0) du_special_partial(R, unused, usc, n, N, M, bpart, rv, ri);
 }
 
-// The last sweep position: its new column, or (an unused atom, jj > 0) the
-// special case begun by du_ba_kernel(jj).
+// The last sweep position: its new column, or (an unused atom, jj > 0, G > 0:
+// the grid of du_ba_kernel) the special case begun by du_ba_kernel(jj).
 __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const int32_t* __restrict__ count,
                                float* __restrict__ D, const float* __restrict__ abuf,
                                const float* __restrict__ anorm, int n, int G, const float* __restrict__ X,
@@ -970,7 +899,8 @@ __global__ void du_tail_kernel(int jj, const uint32_t* __restrict__ perm, const 
     __shared__ float fwred[16];
     du_commit_column(jj, perm, count, D, abuf, anorm, n);
     const int p = (int)perm[jj];
-    if (jj > 0 && count[p] == 0) du_special_finish(p, G, X, D, unused, usc, bitmap, replaced, n, N, M, bpart, fri, fwred);
+    if (G > 0 && jj > 0 && count[p] == 0)
+        du_special_finish(p, G, X, D, unused, usc, bitmap, replaced, n, N, M, bpart, fri, fwred);
 }
 
 // R = X - D Gamma for n <= 1024: one wave per signal, each lane owns 4
@@ -1049,13 +979,13 @@ void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hip
     const int nb = w.blocks;  // grid of the per-atom kernels (grid-stride over the atom's list)
     hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(nb), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
                        w.list_rec, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
-                       M, w.apart, w.bpart, abuf, anorm, w.tickets);
+                       M, w.apart, w.bpart, abuf, anorm);
     hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.apart, abuf, anorm, w.n, nb, 0,
                        w.sweep, X, D, w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart);
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(nb), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
                            w.ptr, w.list_rec, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
-                           w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.tickets, w.sweep);
+                           w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.sweep);
         if (jj + 1 < w.K)
             hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.apart, abuf, anorm, w.n,
                                nb, jj >= 1 ? jj : -1, w.sweep, X, D, w.unused, w.us_counter, w.bitmap, w.replaced,
@@ -1389,14 +1319,27 @@ hipError_t launch_du_sweep(DuWork& w, const float* X, float* D, const int32_t* i
         return hipGetLastError();
     }
     hipLaunchKernelGGL(du_residual_kernel, dim3(RES_BLOCKS), dim3(256), 0, st, X, D, idx, val, cnt, w.n, w.N, w.T, w.R);
+    // per atom: phase A (block partials, or the special case's block maxima),
+    // their reduction (a and ||a||; workgroup 0 finishes the special case),
+    // phase B; the new column is committed by the next phase A or the tail
+    float* abuf = w.dnew;             // 2 x n: a by sweep-position parity
+    float* anorm = w.dnew + 2 * w.n;  // 2 x DU_RB: partial sums of a^2
+    hipLaunchKernelGGL(du_sweep_desc_kernel, dim3((w.K + 255) / 256), dim3(256), 0, st, w.perm_dev, w.count, w.ptr,
+                       w.K, w.sweep);
     const size_t lds_a = (size_t)4 * w.n * sizeof(float);
+    const size_t lds_b = (size_t)w.n * sizeof(float);
     for (int jj = 0; jj < w.K; ++jj) {
         hipLaunchKernelGGL(du_phase_a_kernel, dim3(w.blocks), dim3(256), lds_a, st, jj, w.perm_dev, w.count, w.ptr,
                            w.list_pos, val, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T, M,
-                           w.apart, w.bpart, w.dnew, w.scal, w.tickets);
-        hipLaunchKernelGGL(du_phase_b_kernel, dim3(w.blocks), dim3(256), 0, st, jj, w.perm_dev, w.count, w.ptr,
-                           w.list_pos, val, D, w.R, w.dnew, w.n, w.T, w.tickets + 1);
+                           w.apart, w.bpart, abuf, anorm);
+        hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj, w.apart, abuf, anorm, w.n, w.blocks,
+                           jj, w.sweep, X, D, w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart);
+        hipLaunchKernelGGL(du_phase_b_kernel, dim3(w.blocks), dim3(256), lds_b, st, jj, w.perm_dev, w.count, w.ptr,
+                           w.list_pos, val, D, w.R, abuf, anorm, w.n, w.T);
     }
+    // the last column (the special cases were all finished by the reductions)
+    hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n,
+                       0, X, w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart);
     return hipGetLastError();
 }
 
