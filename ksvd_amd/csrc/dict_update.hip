@@ -16,11 +16,18 @@
 //                          (its block partials of a = E gamma^T), and the
 //                          column commit of the atom at jj - 1, one element per
 //                          workgroup; an unused atom takes the special case
-//                          (sparse errors from R, no GEMM) here;
-//   du_a_reduce_kernel     the fixed-order sum of those partials -> a, ||a||.
-// Above n = 1024 the unpipelined du_phase_a/b kernels do the same work.  All
-// data-dependent branching is device-side (early exits on N_p), so the host
-// loop never synchronises and the sequence is graph-capturable.
+//                          (sparse errors from R, no GEMM) here, as block
+//                          maxima;
+//   du_a_reduce_kernel     the fixed-order sum of those partials -> a, ||a||;
+//                          its workgroup 0 first finishes the special case
+//                          of the previous position.
+// Above n = 1024 the unpipelined du_phase_a_kernel, du_a_reduce_kernel and
+// du_phase_b_kernel do the same work, three launches per atom.  No kernel
+// synchronises across workgroups (a device-scope fence per workgroup costs
+// about 0.1 ms per launch here): every cross-workgroup result goes through a
+// kernel boundary.  All data-dependent branching is device-side (early exits
+// on N_p), so the host loop never synchronises and the sequence is
+// graph-capturable.
 #include <algorithm>
 
 #include "common.hpp"
