@@ -65,12 +65,12 @@ __device__ __forceinline__ int dpp_i(int v) {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
 }
 
-// v += row_bcast of v into the rows of ROWMASK (0xA: rows 1 and 3 take the
-// last lane of rows 0 and 2, row_bcast:15; 0xC: rows 2 and 3 take lane 31,
-// row_bcast:31); the other rows keep v.  One v_add_f32_dpp: the rows it
-// leaves out keep the destination, which is v (the compiler does not fold a
-// -0.0 identity into the add and materialises it instead).  The s_nop gives
-// the two wait states a DPP read needs after the VALU write of v.
+// BCAST = 15: rows 1 and 3 add the last lane of rows 0 and 2 (row_bcast:15,
+// row mask 0xA); BCAST = 31: rows 2 and 3 add lane 31 (row_bcast:31, row
+// mask 0xC); the other rows keep v.  One v_add_f32_dpp: the rows it leaves
+// out keep the destination, which is v (the compiler does not fold a -0.0
+// identity into the add and materialises it instead).  The s_nop gives the
+// two wait states a DPP read needs after the VALU write of v.
 template <int BCAST>
 __device__ __forceinline__ float add_row_bcast(float v) {
     static_assert(BCAST == 15 || BCAST == 31, "row_bcast:15 or row_bcast:31");
