@@ -50,14 +50,16 @@ __device__ __forceinline__ int bf16_swz(int r) { return ((r >> 1) & 3) ^ ((r >> 
 // thread fetches 4 float4s into registers (fetch_tile), later stored to LDS
 // (put_tile), so the next chunk's global loads are in flight while the MFMAs
 // of the current one run.  Mt is the tile's first column (uniform), thread
-// rows are tid/8 + 32 r at 32-bit offsets from it, and `full` (uniform: whole
-// tile in range, rows a multiple of 4 and of BK) drops every bounds test.
+// rows are tid/8 + 32 r at 32-bit offsets from it, and a whole tile (in
+// range, rows a multiple of 4 and of BK) takes no bounds test: FULL = 1 at
+// compile time, FULL = 2 when `full` (uniform) says so at run time.
+template <int FULL>
 __device__ __forceinline__ void fetch_tile(float4* v, const float* __restrict__ Mt, int rows, int nvalid, int t0,
                                            bool vec, bool full) {
     const int tid = threadIdx.x;
     const int row0 = tid >> 3;
     const int t = t0 + (tid & 7) * 4;
-    if (full) {
+    if (FULL == 1 || (FULL == 2 && full)) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) v[r] = *reinterpret_cast<const float4*>(Mt + ((row0 + 32 * r) * rows + t));
         return;
@@ -149,59 +151,22 @@ __device__ __forceinline__ void put_tile(float* __restrict__ s, const float4* v)
     }
 }
 
-// Compiled for WPE workgroups per CU (3 by default: the split form needs 192
-// registers per lane at the compiler's default and would run two -- 3.03 ms
-// against 2.70 ms for the n=256, K=1024, N=10^6 correlation, with a few
-// spilled registers; the FP32 and BF16 forms fit three either way, and so
-// does the three-part split's 48 kB of LDS).  WPE changes the register
-// budget only, not the arithmetic: the autotuner may pick 2.
-template <int FORM, int WPE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
-                                                         const float* __restrict__ B, long colsB,
-                                                         float* __restrict__ C, int tilesA) {
-    // FP32: tiles in LDS as FP32; BF16 forms: as bf16 hi (and lo)
-    constexpr int NF = FORM == 0 ? BM * PITCH : 1;
-    constexpr int NH = FORM == 0 ? 1 : BM * PH;
-    constexpr int NL = FORM >= 2 ? BM * PH : 1;
-    constexpr int NM = FORM == 3 ? BM * PH : 1;
-    __shared__ float sB[NF], sA[NF];
-    __shared__ __align__(16) __bf16 sBh[NH], sAh[NH], sBl[NL], sAl[NL], sBm[NM], sAm[NM];
-    // XCD-aware tile order: workgroups are dispatched round-robin over the 8
-    // XCDs (block b on XCD b % 8), each with its own L2.  Block b works on
-    // XCD x = b % 8 through its r = b / 8-th tile: signal tile
-    // tb = 8 * (r / tilesA) + x and atom tile r % tilesA, so the tilesA
-    // blocks that share a signal tile run on one XCD and read it from HBM
-    // once (a plain b / tilesA order spreads them over all 8 L2s).  The grid
-    // holds tilesB rounded up to a multiple of 8 signal tiles.
-    const long blk = blockIdx.x;
-    const long r8 = blk >> 3;
-    const int ta = (int)(r8 % tilesA);
-    const long tb = 8 * (r8 / tilesA) + (blk & 7);
-    if (tb * BM >= colsB) return;
-    const long j0 = tb * BM;
-    const int m0 = ta * BN;
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int wi = w & 1, wj = w >> 1;
-    const bool vec = (rows & 3) == 0;
-    const int nB = (int)min((long)BM, colsB - j0), nA = min(BN, colsA - m0);
-    const bool full = vec && rows % BK == 0 && nB == BM && nA == BN;
-    const float* Bt = B + j0 * (long)rows;
-    const float* At = A + (long)m0 * rows;
-
-    f32x16 acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-
-    const int li = lane & 31;
-    const int lk = lane >> 5;
+// The chunk loop of at_times_b_kernel.  The BF16 forms instantiate it for
+// whole tiles (F = 1) and for edge tiles (F = 0) apart: with one loop for
+// both, the compiler merged the two fetch paths and waited for the B tile's
+// loads to land before it issued the A tile's.  The FP32 form keeps one loop
+// with the test at run time (F = 2): its whole-tile instance at three
+// workgroups per CU kept the fetch registers in scratch memory.
+template <int FORM, int F>
+__device__ __forceinline__ void chunk_loop(f32x16 (&acc)[2][2], const float* __restrict__ Bt,
+                                           const float* __restrict__ At, int rows, int nB, int nA, bool vec, bool full,
+                                           float* __restrict__ sB, float* __restrict__ sA, __bf16* __restrict__ sBh,
+                                           __bf16* __restrict__ sAh, __bf16* __restrict__ sBl, __bf16* __restrict__ sAl,
+                                           __bf16* __restrict__ sBm, __bf16* __restrict__ sAm, int wi, int wj,
+                                           int li, int lk) {
     float4 pb[4], pa[4];
-    fetch_tile(pb, Bt, rows, nB, 0, vec, full);
-    fetch_tile(pa, At, rows, nA, 0, vec, full);
+    fetch_tile<F>(pb, Bt, rows, nB, 0, vec, full);
+    fetch_tile<F>(pa, At, rows, nA, 0, vec, full);
     for (int t0 = 0; t0 < rows; t0 += BK) {
         if constexpr (FORM == 0) {
             put_tile(sB, pb);
@@ -212,8 +177,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
         }
         __syncthreads();
         if (t0 + BK < rows) {
-            fetch_tile(pb, Bt, rows, nB, t0 + BK, vec, full);
-            fetch_tile(pa, At, rows, nA, t0 + BK, vec, full);
+            fetch_tile<F>(pb, Bt, rows, nB, t0 + BK, vec, full);
+            fetch_tile<F>(pa, At, rows, nA, t0 + BK, vec, full);
         }
         if constexpr (FORM == 0) {
 #pragma unroll
@@ -269,6 +234,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
         }
         __syncthreads();
     }
+}
+
+// Compiled for WPE workgroups per CU (3 by default: the split form needs 192
+// registers per lane at the compiler's default and would run two -- 3.03 ms
+// against 2.70 ms for the n=256, K=1024, N=10^6 correlation, with a few
+// spilled registers; the FP32 and BF16 forms fit three either way, and so
+// does the three-part split's 48 kB of LDS).  WPE changes the register
+// budget only, not the arithmetic: the autotuner may pick 2.
+template <int FORM, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void at_times_b_kernel(const float* __restrict__ A, int rows, int colsA,
+                                                         const float* __restrict__ B, long colsB,
+                                                         float* __restrict__ C, int tilesA) {
+    // FP32: tiles in LDS as FP32; BF16 forms: as bf16 hi (and lo)
+    constexpr int NF = FORM == 0 ? BM * PITCH : 1;
+    constexpr int NH = FORM == 0 ? 1 : BM * PH;
+    constexpr int NL = FORM >= 2 ? BM * PH : 1;
+    constexpr int NM = FORM == 3 ? BM * PH : 1;
+    __shared__ float sB[NF], sA[NF];
+    __shared__ __align__(16) __bf16 sBh[NH], sAh[NH], sBl[NL], sAl[NL], sBm[NM], sAm[NM];
+    // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+    // XCDs (block b on XCD b % 8), each with its own L2.  Block b works on
+    // XCD x = b % 8 through its r = b / 8-th tile: signal tile
+    // tb = 8 * (r / tilesA) + x and atom tile r % tilesA, so the tilesA
+    // blocks that share a signal tile run on one XCD and read it from HBM
+    // once (a plain b / tilesA order spreads them over all 8 L2s).  The grid
+    // holds tilesB rounded up to a multiple of 8 signal tiles.
+    const long blk = blockIdx.x;
+    const long r8 = blk >> 3;
+    const int ta = (int)(r8 % tilesA);
+    const long tb = 8 * (r8 / tilesA) + (blk & 7);
+    if (tb * BM >= colsB) return;
+    const long j0 = tb * BM;
+    const int m0 = ta * BN;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int wi = w & 1, wj = w >> 1;
+    const bool vec = (rows & 3) == 0;
+    const int nB = (int)min((long)BM, colsB - j0), nA = min(BN, colsA - m0);
+    const bool full = vec && rows % BK == 0 && nB == BM && nA == BN;
+    const float* Bt = B + j0 * (long)rows;
+    const float* At = A + (long)m0 * rows;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+
+    const int li = lane & 31;
+    const int lk = lane >> 5;
+    if (FORM == 0)
+        chunk_loop<FORM, 2>(acc, Bt, At, rows, nB, nA, vec, full, sB, sA, sBh, sAh, sBl, sAl, sBm, sAm, wi, wj, li, lk);
+    else if (full)
+        chunk_loop<FORM, 1>(acc, Bt, At, rows, nB, nA, vec, full, sB, sA, sBh, sAh, sBl, sAl, sBm, sAm, wi, wj, li, lk);
+    else
+        chunk_loop<FORM, 0>(acc, Bt, At, rows, nB, nA, vec, full, sB, sA, sBh, sAh, sBl, sAl, sBm, sAm, wi, wj, li, lk);
 
     // Accumulator register r of lane l holds MFMA row 8*(r/4) + 4*(l/32) + r%4
     // (a signal) and column l%32 (an atom); offsets from the tile's corner.
