@@ -45,7 +45,10 @@ def workload(cfg, N, seed=None):
 
 # ---- GEMM / normcols -----------------------------------------------------------------
 @pytest.mark.parametrize("form,tol", [("fp32", 2e-5), ("bf16", 1e-2), ("bf16x3", 1e-4), ("bf16x6", 2e-5)])
-@pytest.mark.parametrize("rows,colsA,colsB", [(64, 256, 1000), (240, 512, 777), (37, 130, 301)])
+# whole tiles plus edge tiles of either operand, all-edge shapes (rows not a
+# multiple of the 32-deep chunk), and a partial atom tile beside whole ones
+@pytest.mark.parametrize("rows,colsA,colsB", [(64, 256, 1000), (240, 512, 777), (37, 130, 301), (256, 200, 1100),
+                                              (32, 128, 2053)])
 def test_at_times_b_forms(o64, form, tol, rows, colsA, colsB):
     say("gemm", form, rows, colsA, colsB)
     rng = np.random.default_rng(rows + colsA)
