@@ -175,9 +175,11 @@ int ensure_du(ksvd_handle* h) {
     TRY(alloc(h, &w.list_rec, (size_t)c.N * c.T, "list_rec"));
     TRY(alloc(h, &w.R, (size_t)c.n * c.N, "R"));
     w.blocks = h->tune.du_blocks > 0 ? h->tune.du_blocks : ksvd::kDuBlocks;
-    TRY(alloc(h, &w.apart, (size_t)ksvd::kDuBlocksMax * ((c.n + 3) & ~3), "apart"));
-    TRY(alloc(h, &w.bpart, (size_t)2 * ksvd::kDuBlocksMax, "bpart"));
+    TRY(alloc(h, &w.apart, (size_t)2 * ksvd::kDuBlocksMax * ((c.n + 3) & ~3), "apart"));
+    TRY(alloc(h, &w.bpart, (size_t)8 * ksvd::kDuBlocksMax, "bpart"));
     TRY(alloc(h, &w.dnew, (size_t)2 * c.n + 32, "dnew"));
+    TRY(alloc(h, &w.dacc, (size_t)3 * ksvd::du_set_words((c.n + 3) & ~3), "dacc"));
+    TRY(alloc(h, &w.dstate, 2, "dstate"));
     TRY(alloc(h, &w.unused, c.N, "unused"));
     TRY(alloc(h, &w.us_counter, 1, "us_counter"));
     TRY(alloc(h, &w.bitmap, c.N, "bitmap"));

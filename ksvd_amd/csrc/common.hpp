@@ -264,6 +264,9 @@ hipError_t launch_compact_to_slotmap(const int32_t* idx, const float* val, const
 // Dictionary update / error / clear (dict_update.hip)
 constexpr int kDuBlocks = 768;      // default workgroups of the per-atom phase kernels (one block partial each)
 constexpr int kDuBlocksMax = 1024;  // the most the scratch (apart, bpart) is sized for
+constexpr int kDuBuckets = 8;       // buckets of an accumulator set of the pipelined sweep (n <= 1024)
+// int64 words of one accumulator set: kDuBuckets x 2 words x np4, a flag word, padding
+__host__ __device__ constexpr long du_set_words(int np4) { return 2L * kDuBuckets * np4 + 2; }
 struct DuWork {
     int n, K, N, T, max_signals, use_thresh;
     double mu_thresh;
@@ -277,9 +280,13 @@ struct DuWork {
                          // unused), 0}
     float* R;            // n x N restricted residuals X - D Gamma
     int blocks;          // workgroups of the per-atom kernels (<= kDuBlocksMax)
-    float* apart;        // blocks x (n rounded up to 4) partial a
-    float* bpart;        // blocks x 2 partial arg-max (special case)
-    float* dnew;         // 2 x n + 2 x 16: a and its partial sums of squares, by sweep-position parity
+    float* apart;        // 2 x kDuBlocksMax x (n rounded up to 4) FP32 block partials of a, by sweep-position
+                         // parity (the pipelined sweep; above n = 1024 the first half, one position)
+    float* bpart;        // 2 x kDuBlocksMax x 4 special-case block records, by sweep-position parity
+    float* dnew;         // 2 x n + 2 x 16: a and its partial sums of squares (above n = 1024), or d_new
+                         // (the pipelined sweep), by sweep-position parity
+    unsigned long long* dacc;  // 3 accumulator sets of du_set_words(np4) (the pipelined sweep)
+    int32_t* dstate;     // 2: the unused list's counter after a sweep position, by position parity
     int32_t* unused;     // N
     int32_t* us_counter; // 1
     int32_t* bitmap;     // N

@@ -8,26 +8,29 @@
 // (idx/val/cnt) and the restricted residuals R = X - D Gamma, formed once per
 // sweep (du_residual4_kernel).  The per-atom lists are built once per sweep in
 // ascending signal order by a two-level counting sort (no order-dependent
-// atomics).  For n <= 1024 the sweep is pipelined by one atom, two launches
+// atomics).  For n <= 1024 the sweep is pipelined by one atom, one launch
 // per atom:
-//   du_ba_kernel(jj)       phase B of the atom at sweep position jj
+//   du_ba_kernel(jj)       every workgroup first forms a = E gamma^T of the
+//                          atom at sweep position jj from the integer
+//                          accumulator set the previous launch filled, and
+//                          d_new = a / ||a||; then phase B of that atom
 //                          (gamma_new = d_new^T (r_s + d_p gamma_s), residual
 //                          update), fused with phase A of the atom at jj + 1
-//                          (its block partials of a = E gamma^T), and the
-//                          column commit of the atom at jj - 1, one element per
-//                          workgroup; an unused atom takes the special case
-//                          (sparse errors from R, no GEMM) here, as block
-//                          maxima;
-//   du_a_reduce_kernel     the fixed-order sum of those partials -> a, ||a||;
-//                          its workgroup 0 first finishes the special case
-//                          of the previous position.
-// Above n = 1024 the unpipelined du_phase_a_kernel, du_a_reduce_kernel and
-// du_phase_b_kernel do the same work, three launches per atom.  No kernel
-// synchronises across workgroups (a device-scope fence per workgroup costs
-// about 0.1 ms per launch here): every cross-workgroup result goes through a
-// kernel boundary.  All data-dependent branching is device-side (early exits
-// on N_p), so the host loop never synchronises and the sequence is
-// graph-capturable.
+//                          (its block partials of a, added as fixed-point
+//                          int64 words: integer sums, so the result does not
+//                          depend on the order of the additions), and the
+//                          column commit of the atom at jj - 1, one element
+//                          per workgroup.  An unused atom takes the special
+//                          case (sparse errors from R, no GEMM): block maxima
+//                          here, the pick and the replacement in the next
+//                          launch.
+// Above n = 1024 the unpipelined du_phase_a_kernel, du_a_reduce_kernel (a
+// fixed-order sum of FP32 block partials) and du_phase_b_kernel do the same
+// work, three launches per atom.  No kernel synchronises across workgroups
+// (a device-scope fence per workgroup costs about 0.1 ms per launch here):
+// every cross-workgroup result goes through a kernel boundary.  All
+// data-dependent branching is device-side (early exits on N_p), so the host
+// loop never synchronises and the sequence is graph-capturable.
 #include <algorithm>
 
 #include "common.hpp"
@@ -590,27 +593,188 @@ __device__ void du_special_finish(int p, int G, const float* __restrict__ X, flo
     }
 }
 
+// ---- the pipelined sweep's cross-workgroup sums (n <= 1024) ---------------
+// Each workgroup adds its block partial of a, component by component, into
+// bucket blockIdx mod kDuBuckets of an accumulator set as two int64 words,
+// h = rint(2^8 f) and l = rint(2^60 (f - 2^-8 h)): exact for
+// 2^-37 <= |f| < 2^44 (smaller parts below 2^-60 are rounded away).  Integer
+// sums do not depend on the order of the additions, so every run gives the
+// same a, and a is the sum of all block partials rounded once.  A partial
+// outside that range (or NaN) sets the set's flag word instead, and the
+// consumers then sum the FP32 block partials (also written, to ap) in block
+// order.  Sets go by sweep position mod 3: launch jj consumes set jj % 3,
+// fills set (jj + 1) % 3 and clears set (jj + 2) % 3; the FP32 partials and
+// the special case's block records by position parity.  Every workgroup of the
+// consuming launch sums the kDuBuckets buckets itself (no separate reduction
+// launch), in the same order, so all of them hold the same a and ||a||.
+constexpr float kDuFixMax = 0x1p44f;
+
+__device__ __forceinline__ void du_publish(float f, int t, int np4, unsigned long long* __restrict__ set,
+                                           float* __restrict__ ap) {
+    ap[t] = f;
+    if (f == 0.f) return;  // adds nothing
+    if (!(fabsf(f) < kDuFixMax)) {
+        atomicOr(set + 2L * kDuBuckets * np4, 1ull);
+        return;
+    }
+    const double fd = (double)f;
+    const double hd = rint(fd * 256.0);
+    const long long h = (long long)hd;
+    const long long l = (long long)rint((fd - hd * 0.00390625) * 0x1p60);  // both steps exact in FP64
+    const int b = blockIdx.x & (kDuBuckets - 1);
+    atomicAdd(set + (2L * b) * np4 + t, (unsigned long long)h);
+    atomicAdd(set + (2L * b + 1) * np4 + t, (unsigned long long)l);
+}
+
+// a into sa (LDS, n floats) and ||a||, the same values in every workgroup;
+// ap: the G FP32 block partials of the position (read after an overflow
+// only).  Uniform control flow; red holds 16 floats.
+__device__ float du_form_a(const unsigned long long* __restrict__ set, const float* __restrict__ ap, int G, int n,
+                           int np4, float* sa, float* red) {
+    const bool ovf = set[2L * kDuBuckets * np4] != 0ull;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        float a;
+        if (!ovf) {
+            long long hs = 0, ls = 0;
+#pragma unroll
+            for (int b = 0; b < kDuBuckets; ++b) {
+                hs += (long long)set[(2L * b) * np4 + t];
+                ls += (long long)set[(2L * b + 1) * np4 + t];
+            }
+            a = (float)((double)hs * 0.00390625 + (double)ls * 0x1p-60);
+        } else {
+            a = 0.f;
+            for (int b = 0; b < G; ++b) a += ap[(long)b * np4 + t];
+        }
+        sa[t] = a;
+    }
+    float sq = 0.f;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) sq += sa[t] * sa[t];
+    return sqrtf(block_sum(sq, red));
+}
+
+// The special case in the pipelined sweep: du_special_partial_at runs in the
+// launch that meets the unused atom, with the counter C of the unused list
+// given and entry sb read as sv (the swap-remove of the previous position's
+// pick, which workgroup 0 of the same launch writes); each block's record
+// {largest error, its entry, ||x||^2 of that entry's signal, 0} goes to bp.
+// The next launch's workgroups pick from the records alone
+// (du_special_pick), so no workgroup but the one applying the pick
+// (du_special_apply, workgroup 0) reads the list entry it rewrites.
+__device__ void du_special_partial_at(const float* __restrict__ R, const float* __restrict__ X,
+                                      const int32_t* __restrict__ unused, int C, int sb, int sv, int n, int N, int M,
+                                      float* __restrict__ bp, float* rv, int* ri) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int G = gridDim.x;
+    const int lim = min(M, N - C);
+    if (lim <= 0) return;  // no candidate: the atom stays as it is
+    float bv = 0.f;        // the reference seeds its reduction with value 0 / index 0
+    int bi = 0;
+    for (int b = blockIdx.x * nw + w; b < lim; b += G * nw) {
+        const long sig = (b == sb) ? sv : unused[b];
+        float e = 0.f;
+        for (int t = lane; t < n; t += 64) {
+            const float r = R[sig * n + t];
+            e += r * r;
+        }
+        e = wave_sum_dpp(e);
+        if (e > bv) { bv = e; bi = b; }  // b ascending per wave: keeps the lowest on ties
+    }
+    if (lane == 0) { rv[w] = bv; ri[w] = bi; }
+    __syncthreads();
+    if (w == 0) {
+        float v = rv[0];
+        int i = ri[0];
+        for (int q = 1; q < nw; ++q)
+            if (rv[q] > v || (rv[q] == v && ri[q] < i)) { v = rv[q]; i = ri[q]; }
+        const long sig = (i == sb) ? sv : unused[i];
+        float xs = 0.f;
+        for (int t = lane; t < n; t += 64) {
+            const float x = X[sig * n + t];
+            xs += x * x;
+        }
+        xs = wave_sum_dpp(xs);
+        if (lane == 0) *reinterpret_cast<float4*>(bp + 4 * blockIdx.x) = make_float4(v, __int_as_float(i), xs, 0.f);
+    }
+}
+
+// The entry that replaces the atom (the largest block maximum, lowest entry
+// among equal ones), or -1: no candidate left, or the pick is a zero signal
+// (x / ||x|| undefined).  G: the grid of the launch that wrote the records.
+// Uniform control flow; ri holds 16 ints, wred 32 floats.
+__device__ int du_special_pick(int G, const float* __restrict__ bp, int C, int N, int M, int* ri, float* wred) {
+    if (min(M, N - C) <= 0) return -1;
+    float v = -1.f, xs = 0.f;
+    int i = 0x7fffffff;
+    for (int q = threadIdx.x; q < G; q += blockDim.x) {
+        const float4 r = *reinterpret_cast<const float4*>(bp + 4 * q);
+        const int iq = __float_as_int(r.y);
+        if (r.x > v || (r.x == v && iq < i)) { v = r.x; i = iq; xs = r.z; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float vo = __shfl_xor(v, off), xo = __shfl_xor(xs, off);
+        const int io = __shfl_xor(i, off);
+        if (vo > v || (vo == v && io < i)) { v = vo; i = io; xs = xo; }
+    }
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) { wred[w] = v; ri[w] = i; wred[16 + w] = xs; }
+    __syncthreads();
+    float vb = wred[0], xb = wred[16];
+    int ib = ri[0];
+    for (int q = 1; q < nw; ++q)
+        if (wred[q] > vb || (wred[q] == vb && ri[q] < ib)) { vb = wred[q]; ib = ri[q]; xb = wred[16 + q]; }
+    __syncthreads();
+    return xb != 0.f ? ib : -1;
+}
+
+// One workgroup: d_p = x / ||x|| for the picked entry's signal, then the
+// swap-remove of the entry (C: the list's counter before the pick),
+// replaced[p] and bitmap (S/DictionaryUpdate.cu:238-302).
+__device__ void du_special_apply(int p, int best, int C, const float* __restrict__ X, float* __restrict__ D,
+                                 int32_t* __restrict__ unused, int32_t* __restrict__ bitmap,
+                                 int32_t* __restrict__ replaced, int n, int N, float* wred) {
+    const long sig = unused[best];
+    float sq = 0.f;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const float x = X[sig * n + t];
+        sq += x * x;
+    }
+    const float nrm = sqrtf(block_sum(sq, wred));
+    for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)p * n + t] = X[sig * n + t] / nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unused[best] = unused[N - C - 1];
+        replaced[p] = 1;
+        bitmap[sig] = 0;
+    }
+}
+
+// Phase A of the atom at sweep position 0 (its set 0 and FP32 block
+// partials, or the special case's block records), and the unused list's
+// counter at the start of the sweep into dstate[0].
 template <int NCH>
-__global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __restrict__ perm,
+__global__ __launch_bounds__(512) void du_a_kernel(const uint32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
                                                    const int4* __restrict__ list_rec, const float* __restrict__ X,
-                                                   float* __restrict__ D, const float* __restrict__ R,
-                                                   int32_t* __restrict__ unused, int32_t* __restrict__ usc,
-                                                   int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
-                                                   int N, int T, int M, float* __restrict__ apart,
-                                                   float* __restrict__ bpart, const float* __restrict__ abuf,
-                                                   const float* __restrict__ anorm) {
+                                                   const float* __restrict__ D, const float* __restrict__ R,
+                                                   const int32_t* __restrict__ unused, const int32_t* __restrict__ usc,
+                                                   int n, int N, int T, int M, float* __restrict__ apart,
+                                                   float* __restrict__ bpart, unsigned long long* __restrict__ dacc,
+                                                   int32_t* __restrict__ dstate) {
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a
     __shared__ float rv[DU_WAVES];
     __shared__ int ri[DU_WAVES];
-    const int p = (int)perm[jj];
+    const int p = (int)perm[0];
     const int Np = count[p];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int G = gridDim.x;
     const bool vec = (n & 3) == 0;
-    if (jj > 0 && blockIdx.x == 0) du_commit_column(jj - 1, perm, count, D, abuf, anorm, n);
+    const int C = *usc;
+    if (blockIdx.x == 0 && threadIdx.x == 0) dstate[0] = C;
     if (Np == 0) {
-        du_special_partial(R, unused, usc, n, N, M, bpart, rv, ri);  // finished by du_a_reduce_kernel(0)
+        du_special_partial_at(R, X, unused, C, -1, 0, n, N, M, bpart, rv, ri);  // picked by du_ba_kernel(1)
         return;
     }
     // a = sum_{s in I_p} gamma_s (r_s + d_p gamma_s)
@@ -658,7 +822,7 @@ __global__ __launch_bounds__(512) void du_a_kernel(int jj, const uint32_t* __res
         float sacc = 0.f;
 #pragma unroll
         for (int q = 0; q < DU_WAVES; ++q) sacc += pf[q * 256 * NCH + t];
-        apart[(long)blockIdx.x * np4 + t] = sacc;
+        du_publish(sacc, t, np4, dacc, apart + (long)blockIdx.x * np4);
     }
 }
 
@@ -752,28 +916,30 @@ __global__ __launch_bounds__(512) void du_a_reduce_kernel(int jj, const float* _
 // Phase B of the atom at sweep position jj fused with phase A of the atom at
 // jj + 1 (the pipelined sweep).  Phase B (S/Multiplication.cu:198-225): for
 // s in I_p, e = r_s + d_p gamma_s, gamma_new = d_new^T e, r_s <- e - d_new
-// gamma_new, with d_new = a / ||a|| from du_a_reduce_kernel.  A signal the
-// next atom q also uses contributes gamma_q (r_s + d_q gamma_q) to q's a from
-// its updated residual, right here (the entry record holds gamma_q); every other
-// signal of I_q is unaffected by this update and contributes from R.  Block
-// partials of q's a go to apart (reduced by du_a_reduce_kernel(jj + 1)).
-// An unused atom p (N_p = 0, jj > 0; position 0 is handled by du_a_kernel)
-// takes the special case here, after q's partials are written.
+// gamma_new, with d_new = a / ||a|| formed by every workgroup from set
+// jj % 3.  A signal the next atom q also uses contributes gamma_q (r_s + d_q
+// gamma_q) to q's a from its updated residual, right here (the entry record
+// holds gamma_q); every other signal of I_q is unaffected by this update and
+// contributes from R.  Block partials of q's a go to set (jj + 1) % 3.
+// First, the atom at position jj - 1: its new column (written by workgroup 0
+// of the previous launch to dbuf) is committed, one element per workgroup, or
+// its special case is picked (by every workgroup that needs the unused
+// list's state after it) and applied by workgroup 0.  An unused atom p
+// (N_p = 0, jj > 0; position 0 is handled by du_a_kernel) writes its special
+// case's block records last.
 template <int NCH>
-__global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_t* __restrict__ perm,
-                                                    const int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
-                                                    const int4* __restrict__ list_rec,
+__global__ __launch_bounds__(512) void du_ba_kernel(int jj, const int4* __restrict__ list_rec,
                                                     const float* __restrict__ X, float* __restrict__ val,
                                                     float* __restrict__ D, float* __restrict__ R,
-                                                    int32_t* __restrict__ unused, int32_t* __restrict__ usc,
-                                                    int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int n,
-                                                    int N, int T, int M, float* __restrict__ apart,
-                                                    float* __restrict__ bpart, const float* __restrict__ abuf,
-                                                    const float* __restrict__ anorm,
-                                                    const int4* __restrict__ sweep) {
+                                                    int32_t* __restrict__ unused, int32_t* __restrict__ bitmap,
+                                                    int32_t* __restrict__ replaced, int n, int N, int T, int M,
+                                                    float* __restrict__ apart, float* __restrict__ bpart,
+                                                    unsigned long long* __restrict__ dacc, float* __restrict__ dbuf,
+                                                    int32_t* __restrict__ dstate, const int4* __restrict__ sweep) {
     __shared__ float4 part[DU_WAVES][64 * NCH];  // per-wave partial a of the next atom
-    __shared__ float rv[DU_WAVES];
-    __shared__ int ri[DU_WAVES];
+    __shared__ float sa[256 * NCH];              // a of this atom
+    __shared__ float rv[DU_WAVES], fwred[32];
+    __shared__ int ri[DU_WAVES], fri[16];
     // this position's atom p and the next one q, their list lengths and offsets
     const int4 d0 = sweep[2 * jj], d1 = sweep[2 * jj + 1];
     const int p = d0.x, Np = d0.y, q = d0.w, Nq = d1.x;
@@ -781,15 +947,37 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
     const int G = gridDim.x;
     const bool vec = (n & 3) == 0;
     const int gw = blockIdx.x * DU_WAVES + w, GW = G * DU_WAVES;
-    // the new column of the atom at position jj - 1, one element per block
-    // (thread 0 of blocks 0 .. n-1), so no block carries it alone
-    if (jj > 0 && threadIdx.x == 0 && (int)blockIdx.x < n) {
+    const int np4 = (n + 3) & ~3;
+    const long sw = du_set_words(np4);
+    const long apw = (long)kDuBlocksMax * np4;  // FP32 block partials per position parity
+    // the atom at position jj - 1; C: the unused list's counter after it, and
+    // the entry sb that its pick rewrites to sv
+    int C, sb = -1, sv = 0;
+    if (jj > 0) {
         const int4 dp = sweep[2 * (jj - 1)];
+        C = dstate[(jj - 1) & 1];
         if (dp.y > 0) {
-            const float nrm = du_anorm(jj - 1, anorm);
-            for (int t = blockIdx.x; t < n; t += gridDim.x)
-                D[(long)dp.x * n + t] = abuf[(long)((jj - 1) & 1) * n + t] / nrm;
+            if (threadIdx.x == 0 && (int)blockIdx.x < n)
+                for (int t = blockIdx.x; t < n; t += G) D[(long)dp.x * n + t] = dbuf[(long)((jj - 1) & 1) * n + t];
+        } else if (blockIdx.x == 0 || Np == 0) {
+            const int best = du_special_pick(G, bpart + (long)((jj - 1) & 1) * 4 * kDuBlocksMax, C, N, M, fri, fwred);
+            if (best >= 0) {
+                sb = best;
+                sv = unused[N - C - 1];  // workgroup 0 writes only entry best
+                if (blockIdx.x == 0) du_special_apply(dp.x, best, C, X, D, unused, bitmap, replaced, n, N, fwred);
+                C += 1;
+            }
         }
+        if (blockIdx.x == 0 && threadIdx.x == 0) dstate[jj & 1] = C;
+    } else {
+        C = dstate[0];
+    }
+    // d_new = a / ||a|| (every workgroup; workgroup 0 also keeps it for the commit)
+    float nrm = 1.f;
+    if (Np > 0) {
+        nrm = du_form_a(dacc + (jj % 3) * sw, apart + (jj & 1) * apw, G, n, np4, sa, fwred);
+        if (blockIdx.x == 0)
+            for (int t = threadIdx.x; t < n; t += blockDim.x) dbuf[(long)(jj & 1) * n + t] = sa[t] / nrm;
     }
     float4 dq[NCH], acc[NCH];
 #pragma unroll
@@ -800,13 +988,11 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
     // per wave, in step: a batch of up to FB phase-B entries of p and up to FB
     // phase-A entries of q, all rows loaded before either is used
     constexpr int FB = FQ / NCH > 0 ? FQ / NCH : 1;
-    const float nrm = (Np > 0) ? du_anorm(jj, anorm) : 1.f;
     float4 dv[NCH], dn[NCH];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         dv[c] = (Np > 0) ? ld4(D + (long)p * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 a4 =
-            (Np > 0) ? ld4(abuf + (long)(jj & 1) * n, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 a4 = (Np > 0) ? ld4(sa, 256 * c + 4 * lane, n, vec) : make_float4(0.f, 0.f, 0.f, 0.f);
         dn[c] = make_float4(a4.x / nrm, a4.y / nrm, a4.z / nrm, a4.w / nrm);  // d_new = a / ||a||
     }
     const int bp = d0.z, bq = d1.y;
@@ -881,17 +1067,47 @@ __global__ __launch_bounds__(512) void du_ba_kernel(int jj, int K, const uint32_
         for (int c = 0; c < NCH; ++c) part[w][64 * c + lane] = acc[c];
         __syncthreads();
         // block partial, rows padded to np4 = n rounded up to 4 floats
-        const int np4 = (n + 3) & ~3;
         const float* pf = reinterpret_cast<const float*>(&part[0][0]);
+        unsigned long long* setq = dacc + ((jj + 1) % 3) * sw;
+        float* apq = apart + ((jj + 1) & 1) * apw + (long)blockIdx.x * np4;
         for (int t = threadIdx.x; t < np4; t += blockDim.x) {
             float sacc = 0.f;
 #pragma unroll
             for (int qq = 0; qq < DU_WAVES; ++qq) sacc += pf[qq * 256 * NCH + t];
-            apart[(long)blockIdx.x * np4 + t] = sacc;
+            du_publish(sacc, t, np4, setq, apq);
         }
     }
-    // an unused atom: finished by the next launch (du_a_reduce_kernel(jj + 1), or du_tail_kernel)
-    if (Np == 0 && jj > 0) du_special_partial(R, unused, usc, n, N, M, bpart, rv, ri);
+    // clear the set of position jj + 2 (last read by the previous launch)
+    {
+        unsigned long long* setz = dacc + ((jj + 2) % 3) * sw;
+        for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < sw; i += (long)G * blockDim.x) setz[i] = 0ull;
+    }
+    // an unused atom: picked by the next launch (du_ba_kernel(jj + 1), or du_tail_pipe_kernel)
+    if (Np == 0 && jj > 0)
+        du_special_partial_at(R, X, unused, C, sb, sv, n, N, M, bpart + (long)(jj & 1) * 4 * kDuBlocksMax, rv, ri);
+}
+
+// The last sweep position jj: its new column, or its special case (picked
+// and applied here); then the unused list's counter after the sweep.
+__global__ void du_tail_pipe_kernel(int jj, const int4* __restrict__ sweep, float* __restrict__ D,
+                                    const float* __restrict__ dbuf, int n, int G, const float* __restrict__ X,
+                                    int32_t* __restrict__ unused, int32_t* __restrict__ usc,
+                                    int32_t* __restrict__ bitmap, int32_t* __restrict__ replaced, int N, int M,
+                                    const float* __restrict__ bpart, const int32_t* __restrict__ dstate) {
+    __shared__ int fri[16];
+    __shared__ float fwred[32];
+    const int4 d = sweep[2 * jj];
+    int C = dstate[jj & 1];
+    if (d.y > 0) {
+        for (int t = threadIdx.x; t < n; t += blockDim.x) D[(long)d.x * n + t] = dbuf[(long)(jj & 1) * n + t];
+    } else {
+        const int best = du_special_pick(G, bpart + (long)(jj & 1) * 4 * kDuBlocksMax, C, N, M, fri, fwred);
+        if (best >= 0) {
+            du_special_apply(d.x, best, C, X, D, unused, bitmap, replaced, n, N, fwred);
+            C += 1;
+        }
+    }
+    if (threadIdx.x == 0) *usc = C;
 }
 
 // The last sweep position: its new column, or (an unused atom, jj > 0, G > 0:
@@ -976,30 +1192,25 @@ __global__ void du_sweep_desc_kernel(const uint32_t* __restrict__ perm, const in
 
 template <int NCH>
 void launch_du_atoms(DuWork& w, const float* X, float* D, float* val, int M, hipStream_t st) {
-    float* abuf = w.dnew;                 // 2 x n: a by sweep-position parity
-    float* anorm = w.dnew + 2 * w.n;      // 2 x DU_RB: partial sums of a^2
-    // sweep position 0: phase A (or the special case) and its reduction; then
-    // per position jj: phase B of jj fused with phase A of jj + 1, and the
-    // reduction of jj + 1 (two launches per atom)
+    // sweep position 0: phase A (or the special case's block records); then per
+    // position jj: phase B of jj fused with phase A of jj + 1 (one launch per
+    // atom); the tail commits the last column.  The accumulator sets start
+    // cleared (a fill kernel: graph-capturable).
+    const long sw = du_set_words((w.n + 3) & ~3);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((6 * sw + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<int32_t*>(w.dacc), 6 * sw, 0);
     hipLaunchKernelGGL(du_sweep_desc_kernel, dim3((w.K + 255) / 256), dim3(256), 0, st, w.perm_dev, w.count, w.ptr,
                        w.K, w.sweep);
     const int nb = w.blocks;  // grid of the per-atom kernels (grid-stride over the atom's list)
-    hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(nb), dim3(512), 0, st, 0, w.perm_dev, w.count, w.ptr,
-                       w.list_rec, X, D, w.R, w.unused, w.us_counter, w.bitmap, w.replaced, w.n, w.N, w.T,
-                       M, w.apart, w.bpart, abuf, anorm);
-    hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, 0, w.apart, abuf, anorm, w.n, nb, 0,
-                       w.sweep, X, D, w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart);
-    for (int jj = 0; jj < w.K; ++jj) {
-        hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(nb), dim3(512), 0, st, jj, w.K, w.perm_dev, w.count,
-                           w.ptr, w.list_rec, X, val, D, w.R, w.unused, w.us_counter, w.bitmap,
-                           w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, abuf, anorm, w.sweep);
-        if (jj + 1 < w.K)
-            hipLaunchKernelGGL(du_a_reduce_kernel, dim3(DU_RB), dim3(512), 0, st, jj + 1, w.apart, abuf, anorm, w.n,
-                               nb, jj >= 1 ? jj : -1, w.sweep, X, D, w.unused, w.us_counter, w.bitmap, w.replaced,
-                               w.N, M, w.bpart);
-    }
-    hipLaunchKernelGGL(du_tail_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.perm_dev, w.count, D, abuf, anorm, w.n,
-                       nb, X, w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart);
+    float* dbuf = w.dnew;     // 2 x n: d_new by sweep-position parity
+    hipLaunchKernelGGL((du_a_kernel<NCH>), dim3(nb), dim3(512), 0, st, w.perm_dev, w.count, w.ptr, w.list_rec, X,
+                       D, w.R, w.unused, w.us_counter, w.n, w.N, w.T, M, w.apart, w.bpart, w.dacc, w.dstate);
+    for (int jj = 0; jj < w.K; ++jj)
+        hipLaunchKernelGGL((du_ba_kernel<NCH>), dim3(nb), dim3(512), 0, st, jj, w.list_rec, X, val, D, w.R,
+                           w.unused, w.bitmap, w.replaced, w.n, w.N, w.T, M, w.apart, w.bpart, w.dacc, dbuf,
+                           w.dstate, w.sweep);
+    hipLaunchKernelGGL(du_tail_pipe_kernel, dim3(1), dim3(256), 0, st, w.K - 1, w.sweep, D, dbuf, w.n, nb, X,
+                       w.unused, w.us_counter, w.bitmap, w.replaced, w.N, M, w.bpart, w.dstate);
 }
 
 // ---- error: S/KSVD.cu:235-262, S/ReductionKernels.cu:257-322 ---------------
