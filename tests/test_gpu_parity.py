@@ -339,6 +339,31 @@ def test_dictionary_update_parity(o64, cfg, N, max_signals):
     eng.close()
 
 
+def test_dictionary_update_outside_fixed_point_range(o64):
+    """Signals and coefficients scaled by 2^25 (exact): block partials of a
+    reach ~2^45 and more, beyond the 2^44 the fixed-point accumulation takes,
+    so the update sums the FP32 block partials instead.  d_new = a / ||a|| and
+    the special case's x / ||x|| do not depend on the scale, so the result is
+    the unscaled oracle's, with the coefficients scaled back."""
+    say("dictionary_update scaled 2^25")
+    X, D, idx, val, cnt, T = _du_inputs(o64, "C0", 3000)
+    n, K, N = D.shape[1], D.shape[0], X.shape[0]
+    sc = np.float32(2.0 ** 25)
+    perm = np.random.default_rng(1).permutation(K).astype(np.uint32)
+    eng = Engine(n, K, N, T, max_signals=5000)
+    Dg, valg = cu(D), cu(val * sc)
+    eng.du_init()
+    eng.updateDictionary(cu(X * sc), Dg, cu(idx, torch.int32), valg, cu(cnt, torch.int32), perm)
+    D1, v1 = host(Dg), host(valg) / sc
+    Dr, vr, st = o64.dict_update(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt, perm,
+                                 max_signals=5000)
+    assert np.all(np.isfinite(D1)) and np.all(np.isfinite(v1))
+    say("D max abs diff", np.abs(D1 - Dr).max(), "val max abs diff", np.abs(v1 - vr).max())
+    np.testing.assert_allclose(D1, Dr, atol=1e-4)
+    np.testing.assert_allclose(v1, vr, atol=1e-4 * max(1.0, np.abs(vr).max()))
+    eng.close()
+
+
 @pytest.mark.parametrize("n", [24, 1100])
 def test_dictionary_update_unused_list_exhausted(o64, n):
     """More unused atoms in one sweep than signals (K = 64 atoms, N = 10
