@@ -339,6 +339,44 @@ def test_dictionary_update_parity(o64, cfg, N, max_signals):
     eng.close()
 
 
+@pytest.mark.parametrize("K,T", [(1, 1), (2, 2), (3, 2)])
+def test_dictionary_update_smallest_sweeps(o64, K, T):
+    """The pipelined sweep at its smallest: K = 1 (the first atom's kernel,
+    one fused launch with no next atom, the tail), K = 2 with signals that use
+    both atoms (the next atom's partial from the freshly updated residual),
+    and K = 3 with atom 2 unused (the special case picked by the tail)."""
+    say("dictionary_update smallest sweep", K, T)
+    n, N = 40, 300
+    rng = np.random.default_rng(K)
+    D = rng.standard_normal((K, n))
+    D = (D / np.linalg.norm(D, axis=1, keepdims=True)).astype(np.float32)
+    X = rng.standard_normal((N, n)).astype(np.float32)
+    idx = np.full((N, T), -1, np.int32)
+    val = np.zeros((N, T), np.float32)
+    cnt = np.zeros(N, np.int32)
+    used = min(K, 2)  # atom 2 of K = 3 stays unused
+    for s_ in range(N):
+        c = 1 + (s_ % min(T, used))
+        atoms = rng.choice(used, size=c, replace=False)
+        idx[s_, :c] = atoms
+        val[s_, :c] = rng.standard_normal(c).astype(np.float32)
+        cnt[s_] = c
+    perm = np.arange(K, dtype=np.uint32)[::-1].copy()
+    eng = Engine(n, K, N, T, max_signals=N)
+    Dg, valg = cu(D), cu(val)
+    eng.du_init()
+    eng.updateDictionary(cu(X), Dg, cu(idx, torch.int32), valg, cu(cnt, torch.int32), perm)
+    D1, v1 = host(Dg), host(valg)
+    Dr, vr, st = o64.dict_update(X.astype(np.float64), D.astype(np.float64), idx, val.astype(np.float64), cnt, perm,
+                                 max_signals=N)
+    if K == 3:
+        assert st["usc"][0] == 1  # the unused atom was replaced
+    say("D max abs diff", np.abs(D1 - Dr).max(), "val max abs diff", np.abs(v1 - vr).max())
+    np.testing.assert_allclose(D1, Dr, atol=1e-5)
+    np.testing.assert_allclose(v1, vr, atol=1e-5 * max(1.0, np.abs(vr).max()))
+    eng.close()
+
+
 def test_dictionary_update_outside_fixed_point_range(o64):
     """Signals and coefficients scaled by 2^25 (exact): block partials of a
     reach ~2^45 and more, beyond the 2^44 the fixed-point accumulation takes,
