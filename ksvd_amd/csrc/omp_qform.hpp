@@ -421,17 +421,19 @@ __global__ __launch_bounds__(64 * W, qform_waves_per_eu(KPL, TT, EPS)) void omp_
         // Wave 0 runs it alone while the workgroup's LDS waits for the next
         // signal, so its chain is kept short: the reciprocals 1/L(r, r) are
         // formed in parallel first (lane r), each step multiplies instead of
-        // dividing, and row r - 1 of L is read while step r runs.
+        // dividing, and the steps are unrolled (guarded by the count), so
+        // the rows of L are read ahead of the chain.
         const int ld = lane < TT ? lane : 0;
         float rhs = (lane < i) ? ys[ld] : 0.f;
         const float rdiag = (lane < i) ? 1.0f / Ls[Lp(ld, ld)] : 0.f;
-        float lrow = (i >= 1 && lane < i - 1) ? Ls[Lp(i - 1, ld)] : 0.f;  // L(r, lane) of the current step
-        for (int r = i - 1; r >= 0; --r) {
-            const float lcur = lrow;
-            if (r >= 1) lrow = (lane < r - 1) ? Ls[Lp(r - 1, ld)] : 0.f;
-            const float gr = readlane(rhs, r) * readlane(rdiag, r);
-            if (lane == r) sp.gam = gr;
-            if (lane < r) rhs = rhs - lcur * gr;
+#pragma unroll
+        for (int r = TT - 1; r >= 0; --r) {
+            if (r < i) {
+                const float lrow = (lane < r) ? Ls[Lp(r, ld)] : 0.f;  // L(r, lane)
+                const float gr = readlane(rhs, r) * readlane(rdiag, r);
+                if (lane == r) sp.gam = gr;
+                if (lane < r) rhs = rhs - lrow * gr;
+            }
         }
         // streaming stores: the results are not re-read by this kernel and
         // should not displace G in L2
