@@ -85,7 +85,7 @@ def test_du_grid_changes_only_rounding():
     wl = synth.make("C1", N=6000)
     X, D0 = wl.X.to(DEV).contiguous(), wl.D_init.to(DEV).contiguous()
     outs = []
-    for blocks in (256, 512, 1024):
+    for blocks in (256, 128, 512, 1024):
         eng = Engine(240, 512, 6000, 16, perm_seed=3)
         eng.set_tuning(du_blocks=blocks)
         D, idx, val, cnt, rmse = eng.ExecuteIterations(X, D0, 2)
